@@ -1,0 +1,274 @@
+"""MI355X conservative-advancement CCD engine -- Python binding of the C ABI.
+
+The engine lives in c2a_amd/lib/libc2a_amd.so (hand-written HIP for gfx950
+plus the host-side builder); this module only marshals arrays through the C
+ABI declared in include/c2a_amd.h.  There is no CPU fallback: if the library
+is missing, loading fails with an error telling how to build it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libc2a_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "c2a_amd.h")
+
+OK, ERR_BAD_ARG, ERR_OOM, ERR_HIP, ERR_NO_DEVICE, ERR_UNSUPPORTED, ERR_EMPTY_MODEL = range(7)
+FLAG_ITER_CAP, FLAG_TRANSLATION_ONLY, FLAG_STACK_OVERFLOW = 1, 2, 4
+
+
+class C2AError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("c2a status %d: %s" % (code, msg))
+        self.code = code
+
+
+class Query(C.Structure):
+    _fields_ = [("pose", (C.c_double * 12) * 4), ("mesh", C.c_int32 * 2),
+                ("seed_tri", C.c_int32 * 2)]
+
+
+class Params(C.Structure):
+    _fields_ = [("tol_d", C.c_double), ("tol_t", C.c_double), ("max_iter", C.c_int32),
+                ("with_contacts", C.c_int32), ("contact_scale", C.c_double),
+                ("contact_offset", C.c_double), ("spec_depth", C.c_int32),
+                ("groups_per_wave", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("toc", C.c_double), ("distance", C.c_double), ("mint", C.c_double),
+                ("R_toc", C.c_double * 9), ("T_toc", C.c_double * 3), ("p1", C.c_double * 3),
+                ("p2", C.c_double * 3), ("bv_tests", C.c_int64), ("tri_tests", C.c_int64),
+                ("numCA", C.c_int32), ("collisionfree", C.c_int32), ("num_contact", C.c_int32),
+                ("last_tri", C.c_int32 * 2), ("flags", C.c_int32)]
+
+
+class StepQuery(C.Structure):
+    _fields_ = [("R1", C.c_double * 9), ("T1", C.c_double * 3), ("R2", C.c_double * 9),
+                ("T2", C.c_double * 3), ("pose_s", (C.c_double * 12) * 2),
+                ("pose_e", (C.c_double * 12) * 2), ("mesh", C.c_int32 * 2),
+                ("seed_tri", C.c_int32 * 2), ("numCA", C.c_int32), ("pad", C.c_int32),
+                ("mint_prev", C.c_double), ("UpboundTOC", C.c_double)]
+
+
+class StepResult(C.Structure):
+    _fields_ = [("distance", C.c_double), ("mint", C.c_double), ("abs_err", C.c_double),
+                ("rel_err", C.c_double), ("p1", C.c_double * 3), ("p2", C.c_double * 3),
+                ("bv_tests", C.c_int64), ("tri_tests", C.c_int64), ("last_tri", C.c_int32 * 2),
+                ("flags", C.c_int32), ("pad", C.c_int32)]
+
+
+RESULT_DTYPE = np.dtype([("toc", "f8"), ("distance", "f8"), ("mint", "f8"), ("R_toc", "f8", 9),
+                         ("T_toc", "f8", 3), ("p1", "f8", 3), ("p2", "f8", 3),
+                         ("bv_tests", "i8"), ("tri_tests", "i8"), ("numCA", "i4"),
+                         ("collisionfree", "i4"), ("num_contact", "i4"), ("last_tri", "i4", 2),
+                         ("flags", "i4")])
+assert RESULT_DTYPE.itemsize == C.sizeof(Result)
+
+_lib = None
+
+# Every extern "C" entry point of include/c2a_amd.h.
+EXPORTS = ["c2a_params_default", "c2a_mesh_create", "c2a_mesh_destroy", "c2a_mesh_info",
+           "c2a_mesh_tri_ids", "c2a_mesh_export", "c2a_ccd_batch", "c2a_batch_create",
+           "c2a_batch_run", "c2a_batch_fetch", "c2a_batch_destroy", "c2a_batch_last_ms",
+           "c2a_ccd_step", "c2a_contact_count", "c2a_motion_prepare", "c2a_motion_pose",
+           "c2a_status_string", "c2a_last_error"]
+
+
+def lib():
+    """Loads libc2a_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("c2a_amd native library missing at %s; build it with "
+                           "`python -c 'import __graft_entry__ as g; g.build()'` or "
+                           "`python c2a_amd/build.py`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, dp, ip, i64 = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_int64
+    L.c2a_params_default.argtypes = [C.POINTER(Params)]
+    L.c2a_params_default.restype = None
+    L.c2a_mesh_create.argtypes = [dp, ip, ip, i64, C.POINTER(vp)]
+    L.c2a_mesh_destroy.argtypes = [vp]
+    L.c2a_mesh_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), ip]
+    L.c2a_mesh_tri_ids.argtypes = [vp, ip]
+    L.c2a_mesh_export.argtypes = [vp, dp, dp]
+    L.c2a_ccd_batch.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Query), i64,
+                                C.POINTER(Params), vp, vp]
+    L.c2a_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Query), i64,
+                                   C.POINTER(Params), C.POINTER(vp)]
+    L.c2a_batch_run.argtypes = [vp, vp]
+    L.c2a_batch_fetch.argtypes = [vp, vp, vp]
+    L.c2a_batch_destroy.argtypes = [vp]
+    L.c2a_batch_last_ms.argtypes = [vp, C.POINTER(C.c_float)]
+    L.c2a_ccd_step.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(StepQuery), i64,
+                               C.POINTER(StepResult), vp]
+    L.c2a_contact_count.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, dp, i64, ip, vp]
+    L.c2a_motion_prepare.argtypes = [dp, dp, dp]
+    L.c2a_motion_pose.argtypes = [dp, C.c_double, dp, dp]
+    L.c2a_status_string.argtypes = [C.c_int]
+    L.c2a_status_string.restype = C.c_char_p
+    L.c2a_last_error.argtypes = []
+    L.c2a_last_error.restype = C.c_char_p
+    for name in EXPORTS:
+        f = getattr(L, name)
+        if f.restype is C.c_int or name not in ("c2a_params_default", "c2a_status_string",
+                                                "c2a_last_error"):
+            if name not in ("c2a_params_default", "c2a_status_string", "c2a_last_error"):
+                f.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != OK:
+        raise C2AError(rc, lib().c2a_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def default_params(**kw):
+    p = Params()
+    lib().c2a_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Mesh:
+    """A triangle soup with its RSS hierarchy (C2A_Model after EndModel)."""
+
+    def __init__(self, tris, ids=None, vidx=None):
+        tris = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+        self.ntri = tris.shape[0]
+        ida = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        vda = None if vidx is None else np.ascontiguousarray(vidx, dtype=np.int32).reshape(-1)
+        h = C.c_void_p()
+        _check(lib().c2a_mesh_create(_dp(tris), None if ida is None else _ip(ida),
+                                     None if vda is None else _ip(vda), self.ntri, C.byref(h)))
+        self.handle = h
+        n, b, d = C.c_int64(), C.c_int64(), C.c_int32()
+        _check(lib().c2a_mesh_info(h, C.byref(n), C.byref(b), C.byref(d)))
+        self.nbv, self.max_depth = b.value, d.value
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                lib().c2a_mesh_destroy(self.handle)
+            except Exception:
+                pass
+            self.handle = None
+
+    def export(self):
+        """(nodes (nbv, 32) float64, tris (ntri, 9) float64) in build order."""
+        nodes = np.zeros((self.nbv, 32))
+        tris = np.zeros((self.ntri, 9))
+        _check(lib().c2a_mesh_export(self.handle, _dp(nodes), _dp(tris)))
+        return nodes, tris
+
+    def tri_ids(self):
+        ids = np.zeros(self.ntri, dtype=np.int32)
+        _check(lib().c2a_mesh_tri_ids(self.handle, _ip(ids)))
+        return ids
+
+
+def _mesh_table(meshes):
+    arr = (C.c_void_p * len(meshes))(*[m.handle for m in meshes])
+    return arr
+
+
+def make_queries(P00, P01, P10, P11, mesh_pairs=None, seeds=None):
+    """Packs (nq,12) pose arrays into a ctypes Query array."""
+    nq = len(P00)
+    qs = (Query * nq)()
+    buf = np.ctypeslib.as_array(qs)
+    poses = np.stack([P00, P01, P10, P11], axis=1).astype(np.float64)
+    buf["pose"] = poses
+    buf["mesh"] = 0 if mesh_pairs is None else np.asarray(mesh_pairs, dtype=np.int32)
+    if mesh_pairs is None:
+        buf["mesh"][:, 1] = 1
+    buf["seed_tri"] = 0 if seeds is None else np.asarray(seeds, dtype=np.int32)
+    return qs
+
+
+def ccd_batch(meshes, queries, params=None, stream=None):
+    """Runs a query batch on the GPU; returns a structured array (RESULT_DTYPE)."""
+    nq = len(queries)
+    out = np.zeros(nq, dtype=RESULT_DTYPE)
+    p = params if params is not None else default_params()
+    _check(lib().c2a_ccd_batch(_mesh_table(meshes), len(meshes), queries, nq, C.byref(p),
+                               out.ctypes.data_as(C.c_void_p), stream))
+    return out
+
+
+class Batch:
+    """Device-resident batch: upload once, run many times (benchmarking)."""
+
+    def __init__(self, meshes, queries, params=None):
+        self._meshes = meshes
+        self.nq = len(queries)
+        p = params if params is not None else default_params()
+        h = C.c_void_p()
+        _check(lib().c2a_batch_create(_mesh_table(meshes), len(meshes), queries, self.nq,
+                                      C.byref(p), C.byref(h)))
+        self.handle = h
+
+    def run(self, stream=None):
+        _check(lib().c2a_batch_run(self.handle, stream))
+
+    def fetch(self, stream=None):
+        out = np.zeros(self.nq, dtype=RESULT_DTYPE)
+        _check(lib().c2a_batch_fetch(self.handle, out.ctypes.data_as(C.c_void_p), stream))
+        return out
+
+    def last_ms(self):
+        ms = C.c_float()
+        _check(lib().c2a_batch_last_ms(self.handle, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                lib().c2a_batch_destroy(self.handle)
+            except Exception:
+                pass
+            self.handle = None
+
+
+def ccd_step(meshes, steps):
+    nq = len(steps)
+    out = (StepResult * nq)()
+    _check(lib().c2a_ccd_step(_mesh_table(meshes), len(meshes), steps, nq, out, None))
+    return out
+
+
+def contact_count(meshes, poses, mesh_pairs, thresholds):
+    poses = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 24)
+    mp = np.ascontiguousarray(mesh_pairs, dtype=np.int32).reshape(-1, 2)
+    th = np.ascontiguousarray(thresholds, dtype=np.float64)
+    out = np.zeros(len(th), dtype=np.int32)
+    _check(lib().c2a_contact_count(_mesh_table(meshes), len(meshes), _dp(poses), _ip(mp),
+                                   _dp(th), len(th), _ip(out), None))
+    return out
+
+
+def motion_prepare(pose_start, pose_end):
+    a = np.ascontiguousarray(pose_start, dtype=np.float64)
+    b = np.ascontiguousarray(pose_end, dtype=np.float64)
+    m = np.zeros(24)
+    _check(lib().c2a_motion_prepare(_dp(a), _dp(b), _dp(m)))
+    return m
+
+
+def motion_pose(motion24, t):
+    m = np.ascontiguousarray(motion24, dtype=np.float64)
+    pose, q = np.zeros(12), np.zeros(4)
+    _check(lib().c2a_motion_pose(_dp(m), float(t), _dp(pose), _dp(q)))
+    return pose, q

@@ -1,0 +1,546 @@
+// capi.cpp -- the extern "C" boundary (include/c2a_amd.h).
+//
+// Host responsibilities: build the RSS hierarchy (bvh.cpp), precompute each
+// query's two linear motions (InterpMotion.cpp:148-270; acos runs here),
+// upload, pick the kernel shape, launch on the caller's stream and copy
+// results back.  No exception crosses this file's entry points.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/c2a_amd.h"
+#include "bvh.h"
+#include "c2a_math.h"
+#include "ccd_kernel.h"
+
+using namespace c2a;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return C2A_ERR_HIP;
+}
+
+#define HIP_TRY(expr)                                  \
+  do {                                                 \
+    hipError_t _e = (expr);                            \
+    if (_e != hipSuccess) return hip_fail(_e, #expr);  \
+  } while (0)
+
+int check_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) return fail(C2A_ERR_NO_DEVICE, "no HIP device available");
+  return C2A_OK;
+}
+
+// A device allocation owned by the library.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t n) {
+    if (n <= bytes && p) return C2A_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, n ? n : 1);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    bytes = n;
+    return C2A_OK;
+  }
+};
+
+void unpack_pose(const double* p12, double* R, double* T) {
+  for (int i = 0; i < 9; i++) R[i] = p12[i];
+  for (int i = 0; i < 3; i++) T[i] = p12[9 + i];
+}
+
+}  // namespace
+
+struct c2a_mesh {
+  Bvh host;
+  DevBuf nodes, tris;
+  int64_t ntri = 0;
+};
+
+struct c2a_batch {
+  std::vector<MeshDev> mtab;
+  DevBuf d_mesh, d_q, d_out, d_counter;
+  int64_t nq = 0;
+  KArgs args{};
+  int K = 3, NG = 4, CAP = 48, blocks = 1;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  ~c2a_batch() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+};
+
+namespace {
+
+// Uploads a mesh's hierarchy and triangles the first time a launch needs it.
+int ensure_uploaded(c2a_mesh* m) {
+  if (m->nodes.p) return C2A_OK;
+  size_t nb = m->host.nodes.size() * sizeof(GpuNode), tb = m->host.tris.size() * sizeof(double);
+  int rc;
+  if ((rc = m->nodes.alloc(nb)) || (rc = m->tris.alloc(tb))) return rc;
+  HIP_TRY(hipMemcpy(m->nodes.p, m->host.nodes.data(), nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(m->tris.p, m->host.tris.data(), tb, hipMemcpyHostToDevice));
+  return C2A_OK;
+}
+
+int make_mesh_table(c2a_mesh* const* meshes, int32_t nmesh, std::vector<MeshDev>& tab) {
+  if (!meshes || nmesh <= 0) return fail(C2A_ERR_BAD_ARG, "empty mesh table");
+  tab.resize((size_t)nmesh);
+  for (int i = 0; i < nmesh; i++) {
+    if (!meshes[i]) return fail(C2A_ERR_BAD_ARG, "null mesh in table");
+    int rc = ensure_uploaded(meshes[i]);
+    if (rc) return rc;
+    tab[(size_t)i].nodes = (const GpuNode*)meshes[i]->nodes.p;
+    tab[(size_t)i].tris = (const double*)meshes[i]->tris.p;
+    tab[(size_t)i].ntri = (int32_t)meshes[i]->ntri;
+    tab[(size_t)i].depth = meshes[i]->host.max_depth;
+  }
+  return C2A_OK;
+}
+
+// Traversal stack capacity: the DFS holds at most one pending pair per BVTT
+// level plus the current one, and the BVTT is at most depth1+depth2 deep.
+int pick_cap(const std::vector<MeshDev>& tab, const int32_t* pairs, int64_t n, int stride) {
+  int need = 0;
+  for (int64_t i = 0; i < n; i++) {
+    int a = pairs[i * stride], b = pairs[i * stride + 1];
+    need = std::max(need, tab[(size_t)a].depth + tab[(size_t)b].depth + 2);
+  }
+  if (need <= 48) return 48;
+  if (need <= 96) return 96;
+  return -1;
+}
+
+int pick_shape(const c2a_params* p, int64_t nq, int& K, int& NG) {
+  K = (p && p->spec_depth > 0) ? p->spec_depth : 3;
+  if (K < 1 || K > 5) return fail(C2A_ERR_BAD_ARG, "spec_depth must be 1..5");
+  int maxg = kernel_groups(K);
+  NG = (K <= 2) ? 8 : maxg;
+  (void)nq;
+  return C2A_OK;
+}
+
+int grid_blocks(int K, int NG, int CAP, int64_t nq) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  int cus = 256;
+  size_t lds_cu = 160 * 1024;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+    cus = prop.multiProcessorCount;
+    if (prop.maxSharedMemoryPerMultiProcessor > 0) lds_cu = prop.maxSharedMemoryPerMultiProcessor;
+  }
+  size_t lds = kernel_lds_bytes(K, NG, CAP);
+  int per_cu = lds ? (int)std::max<size_t>(1, lds_cu / lds) : 1;
+  per_cu = std::min(per_cu, 16);
+  int64_t want = (nq + NG - 1) / NG;
+  int64_t cap = (int64_t)cus * per_cu;
+  return (int)std::max<int64_t>(1, std::min(want, cap));
+}
+
+int prepare_queries(const std::vector<MeshDev>& tab, const c2a_query* q, int64_t nq,
+                    std::vector<QueryDev>& out, std::vector<int32_t>& pairs) {
+  out.resize((size_t)nq);
+  pairs.resize((size_t)nq * 2);
+  for (int64_t i = 0; i < nq; i++) {
+    const c2a_query& x = q[i];
+    QueryDev& d = out[(size_t)i];
+    for (int k = 0; k < 2; k++) {
+      int m = x.mesh[k];
+      if (m < 0 || m >= (int)tab.size()) return fail(C2A_ERR_BAD_ARG, "query mesh index out of range");
+      int s = x.seed_tri[k] < 0 ? 0 : x.seed_tri[k];
+      if (s >= tab[(size_t)m].ntri) return fail(C2A_ERR_BAD_ARG, "seed triangle out of range");
+      d.mesh[k] = m;
+      d.seed[k] = s;
+      pairs[(size_t)i * 2 + k] = m;
+      double R0[9], T0[3], R1[9], T1[3];
+      unpack_pose(x.pose[2 * k], R0, T0);
+      unpack_pose(x.pose[2 * k + 1], R1, T1);
+      motion_init(&d.mo[k], R0, T0, R1, T1);
+    }
+  }
+  return C2A_OK;
+}
+
+int fill_args(c2a_batch* b, const c2a_params* p) {
+  c2a_params def;
+  c2a_params_default(&def);
+  if (!p) p = &def;
+  KArgs& a = b->args;
+  memset(&a, 0, sizeof(a));
+  a.meshes = (const MeshDev*)b->d_mesh.p;
+  a.q = (const QueryDev*)b->d_q.p;
+  a.out = (c2a_result*)b->d_out.p;
+  a.counter = (int64_t*)b->d_counter.p;
+  a.nq = b->nq;
+  a.tol_d = p->tol_d;
+  a.tol_t = p->tol_t;
+  a.contact_scale = p->contact_scale;
+  a.contact_offset = p->contact_offset;
+  a.max_iter = p->max_iter;
+  a.with_contacts = p->with_contacts;
+  a.mode = MODE_QUERY;
+  return C2A_OK;
+}
+
+__global__ void zero_counter(int64_t* c) { *c = 0; }
+
+int run_kernel(const KArgs& a, int K, int NG, int CAP, int blocks, hipStream_t s) {
+  // counter cleared by a kernel (graph-capture safe, no memset node)
+  hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, a.counter);
+  HIP_TRY(hipGetLastError());
+  hipError_t e = launch_ccd(a, K, NG, CAP, blocks, s);
+  if (e != hipSuccess) return hip_fail(e, "launch_ccd");
+  return C2A_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void c2a_params_default(c2a_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->tol_d = 1e-4;
+  p->tol_t = 1e-4;
+  p->max_iter = 150;
+  p->with_contacts = 1;
+  p->contact_scale = 2.0;
+  p->contact_offset = 0.001;
+  p->spec_depth = 0;
+  p->groups_per_wave = 0;
+}
+
+int c2a_mesh_create(const double* tri_xyz9, const int32_t* tri_ids, const int32_t* vert_idx3,
+                    int64_t ntri, c2a_mesh** out) {
+  if (!out) return fail(C2A_ERR_BAD_ARG, "null output handle");
+  *out = nullptr;
+  if (ntri <= 0) return fail(C2A_ERR_EMPTY_MODEL, "model has no triangles");
+  if (!tri_xyz9) return fail(C2A_ERR_BAD_ARG, "null triangle array");
+  if (ntri > (1 << 28)) return fail(C2A_ERR_BAD_ARG, "too many triangles");
+  c2a_mesh* m = new (std::nothrow) c2a_mesh();
+  if (!m) return fail(C2A_ERR_OOM, "host allocation failed");
+  try {
+    if (build_bvh(tri_xyz9, tri_ids, vert_idx3, ntri, &m->host) != 0) {
+      delete m;
+      return fail(C2A_ERR_EMPTY_MODEL, "model has no triangles");
+    }
+  } catch (const std::bad_alloc&) {
+    delete m;
+    return fail(C2A_ERR_OOM, "host allocation failed during the build");
+  }
+  m->ntri = ntri;
+  *out = m;
+  return C2A_OK;
+}
+
+int c2a_mesh_destroy(c2a_mesh* m) {
+  delete m;
+  return C2A_OK;
+}
+
+int c2a_mesh_info(const c2a_mesh* m, int64_t* ntri, int64_t* nbv, int32_t* max_depth) {
+  if (!m) return fail(C2A_ERR_BAD_ARG, "null mesh");
+  if (ntri) *ntri = m->ntri;
+  if (nbv) *nbv = (int64_t)m->host.nodes.size();
+  if (max_depth) *max_depth = m->host.max_depth;
+  return C2A_OK;
+}
+
+int c2a_mesh_tri_ids(const c2a_mesh* m, int32_t* ids_out) {
+  if (!m || !ids_out) return fail(C2A_ERR_BAD_ARG, "null argument");
+  memcpy(ids_out, m->host.tri_id.data(), m->host.tri_id.size() * sizeof(int32_t));
+  return C2A_OK;
+}
+
+int c2a_mesh_export(const c2a_mesh* m, double* nodes32, double* tris9) {
+  if (!m) return fail(C2A_ERR_BAD_ARG, "null mesh");
+  if (nodes32) {
+    for (size_t i = 0; i < m->host.nodes.size(); i++) {
+      const GpuNode& n = m->host.nodes[i];
+      double* o = nodes32 + 32 * i;
+      for (int k = 0; k < 9; k++) o[k] = n.R[k];
+      for (int k = 0; k < 3; k++) o[9 + k] = n.Tr[k];
+      o[12] = n.l[0];
+      o[13] = n.l[1];
+      o[14] = n.r;
+      o[15] = n.angR;
+      for (int k = 0; k < 9; k++) o[16 + k] = n.Rloc[k];
+      o[25] = n.size;
+      o[26] = (double)n.first_child;
+      o[27] = (double)n.depth;
+      o[28] = o[29] = o[30] = o[31] = 0.0;
+    }
+  }
+  if (tris9) memcpy(tris9, m->host.tris.data(), m->host.tris.size() * sizeof(double));
+  return C2A_OK;
+}
+
+int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q, int64_t nq,
+                     const c2a_params* params, c2a_batch** out) {
+  if (!out) return fail(C2A_ERR_BAD_ARG, "null output handle");
+  *out = nullptr;
+  if (nq < 0 || (nq > 0 && !q)) return fail(C2A_ERR_BAD_ARG, "bad query array");
+  int rc = check_device();
+  if (rc) return rc;
+  c2a_batch* b = new (std::nothrow) c2a_batch();
+  if (!b) return fail(C2A_ERR_OOM, "host allocation failed");
+  std::vector<QueryDev> hq;
+  std::vector<int32_t> pairs;
+  try {
+    if ((rc = make_mesh_table(meshes, nmesh, b->mtab)) ||
+        (rc = prepare_queries(b->mtab, q, nq, hq, pairs))) {
+      delete b;
+      return rc;
+    }
+  } catch (const std::bad_alloc&) {
+    delete b;
+    return fail(C2A_ERR_OOM, "host allocation failed");
+  }
+  b->nq = nq;
+  if ((rc = pick_shape(params, nq, b->K, b->NG))) {
+    delete b;
+    return rc;
+  }
+  b->CAP = nq ? pick_cap(b->mtab, pairs.data(), nq, 2) : 48;
+  if (b->CAP < 0) {
+    delete b;
+    return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
+  }
+  b->blocks = grid_blocks(b->K, b->NG, b->CAP, nq);
+  if ((rc = b->d_mesh.alloc(b->mtab.size() * sizeof(MeshDev))) ||
+      (rc = b->d_q.alloc((size_t)std::max<int64_t>(nq, 1) * sizeof(QueryDev))) ||
+      (rc = b->d_out.alloc((size_t)std::max<int64_t>(nq, 1) * sizeof(c2a_result))) ||
+      (rc = b->d_counter.alloc(sizeof(int64_t)))) {
+    delete b;
+    return rc;
+  }
+  hipError_t e = hipMemcpy(b->d_mesh.p, b->mtab.data(), b->mtab.size() * sizeof(MeshDev),
+                           hipMemcpyHostToDevice);
+  if (e == hipSuccess && nq)
+    e = hipMemcpy(b->d_q.p, hq.data(), (size_t)nq * sizeof(QueryDev), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e != hipSuccess) {
+    delete b;
+    return hip_fail(e, "batch upload");
+  }
+  fill_args(b, params);
+  *out = b;
+  return C2A_OK;
+}
+
+int c2a_batch_run(c2a_batch* b, void* stream) {
+  if (!b) return fail(C2A_ERR_BAD_ARG, "null batch");
+  if (b->nq == 0) return C2A_OK;
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipEventRecord(b->ev0, s));
+  int rc = run_kernel(b->args, b->K, b->NG, b->CAP, b->blocks, s);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(b->ev1, s));
+  b->timed = true;
+  return C2A_OK;
+}
+
+int c2a_batch_fetch(c2a_batch* b, c2a_result* out, void* stream) {
+  if (!b || (!out && b->nq)) return fail(C2A_ERR_BAD_ARG, "null argument");
+  if (b->nq == 0) return C2A_OK;
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpyAsync(out, b->d_out.p, (size_t)b->nq * sizeof(c2a_result),
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return C2A_OK;
+}
+
+int c2a_batch_last_ms(c2a_batch* b, float* ms) {
+  if (!b || !ms) return fail(C2A_ERR_BAD_ARG, "null argument");
+  if (!b->timed) return fail(C2A_ERR_BAD_ARG, "batch has not run");
+  HIP_TRY(hipEventSynchronize(b->ev1));
+  HIP_TRY(hipEventElapsedTime(ms, b->ev0, b->ev1));
+  return C2A_OK;
+}
+
+int c2a_batch_destroy(c2a_batch* b) {
+  delete b;
+  return C2A_OK;
+}
+
+int c2a_ccd_batch(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q, int64_t nq,
+                  const c2a_params* params, c2a_result* out, void* stream) {
+  if (nq > 0 && !out) return fail(C2A_ERR_BAD_ARG, "null result array");
+  c2a_batch* b = nullptr;
+  int rc = c2a_batch_create(meshes, nmesh, q, nq, params, &b);
+  if (rc) return rc;
+  rc = c2a_batch_run(b, stream);
+  if (!rc) rc = c2a_batch_fetch(b, out, stream);
+  c2a_batch_destroy(b);
+  return rc;
+}
+
+int c2a_ccd_step(c2a_mesh* const* meshes, int32_t nmesh, const c2a_step_query* q, int64_t nq,
+                 c2a_step_result* out, void* stream) {
+  if (nq < 0 || (nq > 0 && (!q || !out))) return fail(C2A_ERR_BAD_ARG, "bad arrays");
+  if (nq == 0) return C2A_OK;
+  int rc = check_device();
+  if (rc) return rc;
+  std::vector<MeshDev> tab;
+  if ((rc = make_mesh_table(meshes, nmesh, tab))) return rc;
+  std::vector<StepDev> hs((size_t)nq);
+  std::vector<int32_t> pairs((size_t)nq * 2);
+  for (int64_t i = 0; i < nq; i++) {
+    const c2a_step_query& x = q[i];
+    StepDev& d = hs[(size_t)i];
+    memset(&d, 0, sizeof(d));
+    for (int k = 0; k < 2; k++) {
+      int m = x.mesh[k];
+      if (m < 0 || m >= nmesh) return fail(C2A_ERR_BAD_ARG, "mesh index out of range");
+      int s = x.seed_tri[k] < 0 ? 0 : x.seed_tri[k];
+      if (s >= tab[(size_t)m].ntri) return fail(C2A_ERR_BAD_ARG, "seed triangle out of range");
+      d.mesh[k] = m;
+      d.seed[k] = s;
+      d.last[k] = s;
+      pairs[(size_t)i * 2 + k] = m;
+      double R0[9], T0[3], R1[9], T1[3];
+      unpack_pose(x.pose_s[k], R0, T0);
+      unpack_pose(x.pose_e[k], R1, T1);
+      motion_init(&d.mo[k], R0, T0, R1, T1);
+    }
+    memcpy(d.R1, x.R1, sizeof(d.R1));
+    memcpy(d.T1, x.T1, sizeof(d.T1));
+    memcpy(d.R2, x.R2, sizeof(d.R2));
+    memcpy(d.T2, x.T2, sizeof(d.T2));
+    d.numCA = x.numCA;
+    d.mint_prev = x.mint_prev;
+    d.UpboundTOC = x.UpboundTOC;
+  }
+  int CAP = pick_cap(tab, pairs.data(), nq, 2);
+  if (CAP < 0) return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
+  int K = 3, NG = kernel_groups(3);
+  DevBuf dm, din, dout, dc;
+  if ((rc = dm.alloc(tab.size() * sizeof(MeshDev))) || (rc = din.alloc((size_t)nq * sizeof(StepDev))) ||
+      (rc = dout.alloc((size_t)nq * sizeof(c2a_step_result))) || (rc = dc.alloc(sizeof(int64_t))))
+    return rc;
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpy(dm.p, tab.data(), tab.size() * sizeof(MeshDev), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(din.p, hs.data(), (size_t)nq * sizeof(StepDev), hipMemcpyHostToDevice));
+  KArgs a;
+  memset(&a, 0, sizeof(a));
+  a.meshes = (const MeshDev*)dm.p;
+  a.step_in = (const StepDev*)din.p;
+  a.step_out = (c2a_step_result*)dout.p;
+  a.counter = (int64_t*)dc.p;
+  a.nq = nq;
+  a.mode = MODE_STEP;
+  a.max_iter = 150;
+  if ((rc = run_kernel(a, K, NG, CAP, grid_blocks(K, NG, CAP, nq), s))) return rc;
+  HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)nq * sizeof(c2a_step_result), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return C2A_OK;
+}
+
+int c2a_contact_count(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                      const int32_t* mesh_pairs, const double* thresholds, int64_t nq,
+                      int32_t* counts_out, void* stream) {
+  if (nq < 0 || (nq > 0 && (!poses || !mesh_pairs || !thresholds || !counts_out)))
+    return fail(C2A_ERR_BAD_ARG, "bad arrays");
+  if (nq == 0) return C2A_OK;
+  int rc = check_device();
+  if (rc) return rc;
+  std::vector<MeshDev> tab;
+  if ((rc = make_mesh_table(meshes, nmesh, tab))) return rc;
+  std::vector<ContactDev> hc((size_t)nq);
+  for (int64_t i = 0; i < nq; i++) {
+    ContactDev& c = hc[(size_t)i];
+    memset(&c, 0, sizeof(c));
+    for (int k = 0; k < 2; k++) {
+      int m = mesh_pairs[2 * i + k];
+      if (m < 0 || m >= nmesh) return fail(C2A_ERR_BAD_ARG, "mesh index out of range");
+      c.mesh[k] = m;
+    }
+    unpack_pose(poses + 24 * i, c.R1, c.T1);
+    unpack_pose(poses + 24 * i + 12, c.R2, c.T2);
+    c.threshold = thresholds[i];
+  }
+  int CAP = pick_cap(tab, mesh_pairs, nq, 2);
+  if (CAP < 0) return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
+  int K = 3, NG = kernel_groups(3);
+  DevBuf dm, din, dout, dc;
+  if ((rc = dm.alloc(tab.size() * sizeof(MeshDev))) ||
+      (rc = din.alloc((size_t)nq * sizeof(ContactDev))) ||
+      (rc = dout.alloc((size_t)nq * sizeof(int32_t))) || (rc = dc.alloc(sizeof(int64_t))))
+    return rc;
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpy(dm.p, tab.data(), tab.size() * sizeof(MeshDev), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(din.p, hc.data(), (size_t)nq * sizeof(ContactDev), hipMemcpyHostToDevice));
+  KArgs a;
+  memset(&a, 0, sizeof(a));
+  a.meshes = (const MeshDev*)dm.p;
+  a.contact_in = (const ContactDev*)din.p;
+  a.counts_out = (int32_t*)dout.p;
+  a.counter = (int64_t*)dc.p;
+  a.nq = nq;
+  a.mode = MODE_CONTACT;
+  if ((rc = run_kernel(a, K, NG, CAP, grid_blocks(K, NG, CAP, nq), s))) return rc;
+  HIP_TRY(hipMemcpyAsync(counts_out, dout.p, (size_t)nq * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return C2A_OK;
+}
+
+int c2a_motion_prepare(const double* pose_start, const double* pose_end, double* motion24) {
+  if (!pose_start || !pose_end || !motion24) return fail(C2A_ERR_BAD_ARG, "null argument");
+  double R0[9], T0[3], R1[9], T1[3];
+  unpack_pose(pose_start, R0, T0);
+  unpack_pose(pose_end, R1, T1);
+  Motion m;
+  motion_init(&m, R0, T0, R1, T1);
+  memcpy(motion24, &m, sizeof(double) * 24);
+  return C2A_OK;
+}
+
+int c2a_motion_pose(const double* motion24, double t, double* pose12, double* quat4) {
+  if (!motion24 || !pose12) return fail(C2A_ERR_BAD_ARG, "null argument");
+  Motion m;
+  memcpy(&m, motion24, sizeof(double) * 24);
+  motion_integrate(m, t, pose12, pose12 + 9, quat4);
+  return C2A_OK;
+}
+
+const char* c2a_status_string(int status) {
+  switch (status) {
+    case C2A_OK: return "ok";
+    case C2A_ERR_BAD_ARG: return "bad argument";
+    case C2A_ERR_OOM: return "out of memory";
+    case C2A_ERR_HIP: return "HIP runtime error";
+    case C2A_ERR_NO_DEVICE: return "no GPU device";
+    case C2A_ERR_UNSUPPORTED: return "unsupported input";
+    case C2A_ERR_EMPTY_MODEL: return "empty model";
+    default: return "unknown status";
+  }
+}
+
+const char* c2a_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+/*
+ * c2a_amd.h -- C ABI of the MI355X conservative-advancement CCD engine.
+ *
+ * This is the drop-in boundary for the reference's CCD path
+ * (EwhaGlab/C2A):
+ *   C2A_Solve               C2A/C2A.h:23-35,   C2A/src/C2A.cpp:2315-2444
+ *   C2A_QueryTimeOfContact  C2A/C2A.h:274-281, C2A/src/C2A.cpp:1987-2146
+ *   C2A_TimeOfContactStep   C2A/src/C2A.cpp:1778-1931 (not declared in C2A.h)
+ *   C2A_QueryContact        C2A/C2A.h:284-289, C2A/src/C2A.cpp:1937-1966
+ *   C2A_Model Begin/AddTri/EndModel  C2A/C2A_Internal.h:57-64,
+ *                                    C2A/src/C2A_PQP.cpp:82-417
+ * The C++ layer in include/c2a/ (C2A.h, C2A_Internal.h, InterpMotion.h,
+ * LinearMath.h) rebuilds the reference's source interface on top of these
+ * entry points.
+ *
+ * Rules: plain C types only; every entry point returns a c2a_status; no
+ * exception crosses the boundary; the caller owns host buffers, the library
+ * owns device buffers; handles are opaque.  Results match the reference
+ * semantics bit for bit (FP64 throughout, same operation order).
+ */
+#ifndef C2A_AMD_H
+#define C2A_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  C2A_OK = 0,
+  C2A_ERR_BAD_ARG = 1,
+  C2A_ERR_OOM = 2,
+  C2A_ERR_HIP = 3,
+  C2A_ERR_NO_DEVICE = 4,
+  C2A_ERR_UNSUPPORTED = 5,
+  C2A_ERR_EMPTY_MODEL = 6   /* PQP_ERR_BUILD_EMPTY_MODEL (C2A_PQP.cpp:352-357) */
+} c2a_status;
+
+typedef struct c2a_mesh c2a_mesh;
+typedef struct c2a_batch c2a_batch;
+
+/* One CCD query: start/end poses of object 1 then object 2, each a
+ * row-major 3x3 rotation followed by the translation (12 doubles).  The
+ * meshes are indices into the mesh table passed with the batch.  seed_tri is
+ * the initial triangle pair that bounds the first distance (the reference's
+ * dres.last_triA/B, C2A.cpp:1812-1813), as indices in the mesh's build
+ * order; -1 means triangle 0 (the model's last_tri after EndModel). */
+typedef struct {
+  double pose[4][12];
+  int32_t mesh[2];
+  int32_t seed_tri[2];
+} c2a_query;
+
+/* Query parameters.  Defaults (c2a_params_default) are C2A_Solve's:
+ * tol_d = tol_t = 1e-4, 150 iterations, contacts counted within
+ * 2*distance + 0.001 of the TOC pose. */
+typedef struct {
+  double tol_d;
+  double tol_t;
+  int32_t max_iter;
+  int32_t with_contacts;
+  double contact_scale;
+  double contact_offset;
+  int32_t spec_depth;       /* speculative subtree depth per round, 0 = auto */
+  int32_t groups_per_wave;  /* queries per wavefront, 0 = auto */
+} c2a_params;
+
+/* Per-query result (C2A_TimeOfContactResult, C2A.h:161-224, plus the
+ * reference's per-model last_tri after the query). */
+typedef struct {
+  double toc;
+  double distance;
+  double mint;
+  double R_toc[9];
+  double T_toc[3];
+  double p1[3];
+  double p2[3];
+  int64_t bv_tests;
+  int64_t tri_tests;
+  int32_t numCA;
+  int32_t collisionfree;
+  int32_t num_contact;
+  int32_t last_tri[2];
+  int32_t flags;            /* C2A_FLAG_* */
+} c2a_result;
+
+enum {
+  C2A_FLAG_ITER_CAP = 1,          /* the 150-iteration cap was hit */
+  C2A_FLAG_TRANSLATION_ONLY = 2,  /* both rotations below 1e-8: the reference
+                                     would take its translation-only branch */
+  C2A_FLAG_STACK_OVERFLOW = 4     /* traversal deeper than the stack (never
+                                     with capacities chosen from mesh depth) */
+};
+
+void c2a_params_default(c2a_params *p);
+
+/* Builds the RSS hierarchy on the host (PQP-style, C2A_Build.cpp); it is
+ * uploaded to the current device the first time a launch uses it.  tri_xyz9: ntri x 9 doubles (p1, p2, p3).  tri_ids / vert_idx3 may be
+ * NULL (ids default to the input order). */
+int c2a_mesh_create(const double *tri_xyz9, const int32_t *tri_ids,
+                    const int32_t *vert_idx3, int64_t ntri, c2a_mesh **out);
+int c2a_mesh_destroy(c2a_mesh *m);
+/* Number of triangles, bounding volumes and maximum depth. */
+int c2a_mesh_info(const c2a_mesh *m, int64_t *ntri, int64_t *nbv, int32_t *max_depth);
+/* Caller ids (AddTri "id") of the triangles in build order. */
+int c2a_mesh_tri_ids(const c2a_mesh *m, int32_t *ids_out);
+/* Copies the built hierarchy back to the host: nodes as nbv x 32 doubles
+ * (R[9], Tr[3], l[2], r, angR, Rloc[9], size, then first_child and depth
+ * stored as doubles), triangles as ntri x 9 doubles in build order. */
+int c2a_mesh_export(const c2a_mesh *m, double *nodes32, double *tris9);
+
+/* Runs nq independent queries on the GPU (C2A_QueryTimeOfContact semantics,
+ * plus C2A_Solve's contact count when params->with_contacts).  Host in/out;
+ * stream may be NULL (the null stream). */
+int c2a_ccd_batch(c2a_mesh *const *meshes, int32_t nmesh, const c2a_query *q, int64_t nq,
+                  const c2a_params *params, c2a_result *out, void *stream);
+
+/* Device-resident batches: inputs uploaded once, run repeatedly, results
+ * fetched on demand (used to time the kernel with inputs already in HBM). */
+int c2a_batch_create(c2a_mesh *const *meshes, int32_t nmesh, const c2a_query *q, int64_t nq,
+                     const c2a_params *params, c2a_batch **out);
+int c2a_batch_run(c2a_batch *b, void *stream);
+int c2a_batch_fetch(c2a_batch *b, c2a_result *out, void *stream);
+int c2a_batch_destroy(c2a_batch *b);
+/* Kernel-only device time of the last c2a_batch_run, in milliseconds,
+ * measured with HIP events on the stream the kernel was launched on. */
+int c2a_batch_last_ms(c2a_batch *b, float *ms);
+
+/* One conservative-advancement step at explicit poses (backs
+ * C2A_TimeOfContactStep).  state_in: numCA, previous mint and UpboundTOC as
+ * the CA loop leaves them; seed: the initial triangle pair.  Writes
+ * distance, mint, abs/rel error, counters and the updated last_tri. */
+typedef struct {
+  double R1[9], T1[3], R2[9], T2[3];   /* current absolute poses */
+  double pose_s[2][12], pose_e[2][12]; /* start/end poses defining the motions */
+  int32_t mesh[2];
+  int32_t seed_tri[2];
+  int32_t numCA;
+  int32_t pad;
+  double mint_prev;
+  double UpboundTOC;
+} c2a_step_query;
+typedef struct {
+  double distance, mint, abs_err, rel_err;
+  double p1[3], p2[3];
+  int64_t bv_tests, tri_tests;
+  int32_t last_tri[2];
+  int32_t flags, pad;
+} c2a_step_result;
+int c2a_ccd_step(c2a_mesh *const *meshes, int32_t nmesh, const c2a_step_query *q, int64_t nq,
+                 c2a_step_result *out, void *stream);
+
+/* Contact enumeration at explicit poses (C2A_QueryContact): counts the
+ * triangle pairs within `threshold` (distance <= threshold). */
+int c2a_contact_count(c2a_mesh *const *meshes, int32_t nmesh, const double *poses /* nq x 2 x 12 */,
+                      const int32_t *mesh_pairs /* nq x 2 */, const double *thresholds, int64_t nq,
+                      int32_t *counts_out, void *stream);
+
+/* Linear motion between two poses, as the kernel integrates it
+ * (CInterpMotion_Linear ctor + velocity(), InterpMotion.cpp:148-270,
+ * 486-513): motion24 = R0[9], T0[3], q0[4] (x,y,z,w), cv[3], axis[3],
+ * angVel, 0.  Host only. */
+int c2a_motion_prepare(const double *pose_start, const double *pose_end, double *motion24);
+/* Pose at t (CInterpMotion_Linear::integrate, InterpMotion.cpp:516-568):
+ * pose12 = R[9] then T[3]; quat4 (optional) = (x,y,z,w). */
+int c2a_motion_pose(const double *motion24, double t, double *pose12, double *quat4);
+
+const char *c2a_status_string(int status);
+/* Thread-local text of the last error, "" if none. */
+const char *c2a_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,186 @@
+/*
+ * c2a_oracle.h -- FP64 CPU oracle for the conservative-advancement CCD path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (EwhaGlab/C2A + the PQP routines it calls).  Only the test suite,
+ * the smoke check and the CPU-baseline leg of the benchmark may load it, and
+ * only as the checker.  The product library never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Where the
+ * arithmetic lives in PQP (GammaUNC/PQP, not vendored, version unpinned) the
+ * published algorithm is restated and the citation names the C2A call site.
+ *
+ * Pinning status of each part (what checks it besides self-consistency) is
+ * listed in the test files and DESIGN.md; in short:
+ *   rect_dist          pinned (brute force over edge pairs + corner/face
+ *                      projections, dense sampling, symmetry)
+ *   tri_dist           pinned (brute force 9 seg-seg + 6 vertex-face, sampling)
+ *   motion / integrate pinned (closed forms t=0, t=1, constant-axis slerp)
+ *   motion bounds      pinned (closed form + sampling with R0 = I)
+ *   bvh build          pinned by invariants (containment, 2n-1 nodes,
+ *                      permutation, angular radius closed form); the exact
+ *                      tree shape depends on PQP's Meigen -> parity unpinned
+ *   toc step / CA loop pinned by brute-force distance in exact mode,
+ *                      scripted state machine, dense-sampling conservativeness
+ *   contact count      pinned (brute-force count of pairs within threshold)
+ */
+#ifndef C2A_ORACLE_H
+#define C2A_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Triangle record (PQP Tri + C2A_Tri index triple, C2A_Tri.h:46-52). */
+typedef struct {
+  double p[3][3];   /* vertices p1,p2,p3 */
+  int id;           /* caller id (AddTri "i") */
+  int idx[3];       /* vertex indices (AddTri i1,i2,i3) */
+} c2o_tri;
+
+/* RSS bounding volume (PQP BV + C2A_BV extras used on the CCD path,
+ * C2A_BV.h:33-77). R/Tr are parent-relative after the build. */
+typedef struct {
+  double R[3][3];
+  double Tr[3];
+  double l[2];
+  double r;
+  double R_loc[3][3];   /* model-frame orientation snapshot (C2A_Build.cpp:439) */
+  double angR;          /* max |vertex| from the model origin (C2A_BV.cpp:71-111) */
+  double size;          /* PQP BV::GetSize for RSS: sqrt(l0^2+l1^2) + 2r */
+  int first_child;      /* >=0: child bv pair index; <0: leaf, tri = -fc-1 */
+  int depth;
+} c2o_bv;
+
+typedef struct {
+  int ntri;
+  c2o_tri *tris;        /* reordered by the build (split_tris swaps) */
+  int nbv;
+  c2o_bv *bvs;
+  int last_tri;         /* PQP_Model::last_tri as an index into tris */
+} c2o_model;
+
+/* Linear motion (CInterpMotion_Linear, InterpMotion.cpp:486-568). */
+typedef struct {
+  double R0[3][3], T0[3];
+  double R1[3][3], T1[3];
+  double q0[4];         /* quaternion of R0, (x,y,z,w) */
+  double cv[3];         /* T1 - T0 */
+  double axis[3];       /* body-frame unit axis */
+  double angVel;        /* rotation angle over t in [0,1] */
+} c2o_motion;
+
+/* Per-query result, mirroring C2A_TimeOfContactResult (C2A.h:161-224). */
+typedef struct {
+  double toc;
+  double distance;
+  double mint;
+  double p1[3], p2[3];
+  double R_toc[3][3], T_toc[3];
+  double abs_err, rel_err, UpboundTOC;
+  int numCA;
+  int collisionfree;
+  int num_contact;
+  long long num_bv_tests;
+  long long num_tri_tests;
+  int last_triA, last_triB;   /* seed pair (caller-owned in the reference) */
+  int out_last_triA, out_last_triB; /* model last_tri after the query */
+  int flags;                  /* C2O_FLAG_* */
+} c2o_result;
+
+enum {
+  C2O_FLAG_ITER_CAP = 1,        /* hit the 150-iteration cap (C2A.cpp:2079) */
+  C2O_FLAG_TRANSLATION_ONLY = 2 /* both angVel < 1e-8: translation CA path,
+                                   not restated (C2A.cpp:1818-1852) */
+};
+
+/* Contact record produced by the contact enumeration (ContactF, C2A.h:116-153). */
+typedef struct {
+  int triA, triB;        /* indices into the reordered tris arrays */
+  int typeA, typeB;      /* 1 vertex, 2 edge, 3 face */
+  int fidA[3], fidB[3];
+  double PA[3], PB[3];
+  double distance;
+} c2o_contact;
+
+/* sin/cos used by integrate: 0 = the hand-written reduction + polynomial
+ * shared with the GPU path (default), 1 = libm. */
+void c2o_set_libm_trig(int use_libm);
+void c2o_sincos(double x, double *s, double *c);
+
+/* ---- primitives (exported for the pinning tests) ---- */
+double c2o_rect_dist(const double Rab[3][3], const double Tab[3],
+                     const double a[2], const double b[2],
+                     double P[3], double Q[3], double S[3]);
+double c2o_tri_dist(double P[3], double Q[3], const double S[3][3],
+                    const double T[3][3], int *typeA, int *fidA,
+                    int *typeB, int *fidB);
+void c2o_quat_from_matrix(const double R[3][3], double q[4]);
+void c2o_matrix_from_quat(const double q[4], double R[3][3]);
+void c2o_motion_init(c2o_motion *m, const double R0[3][3], const double T0[3],
+                     const double R1[3][3], const double T1[3]);
+void c2o_motion_integrate(const c2o_motion *m, double t, double R[3][3],
+                          double T[3], double qua7[7]);
+double c2o_motion_bound(const c2o_motion *m, double angR, const double N[3]);
+double c2o_motion_bound_leaf(const c2o_motion *m, double angR, const double S[3]);
+
+/* ---- model build (C2A_Model::BeginModel/AddTri/EndModel) ---- */
+int c2o_model_build(c2o_model *m, int n, const double *xyz9, const int *ids,
+                    const int *idx3);
+void c2o_model_free(c2o_model *m);
+void c2o_jacobi_eigen(const double A[3][3], double V[3][3], double d[3]);
+
+/* ---- the CCD path ---- */
+typedef struct {
+  double tol_d, tol_t;
+  int max_iter;          /* 150 in the reference */
+} c2o_params;
+
+/* One C2A_TimeOfContactStep at the given absolute poses.  res must hold
+ * numCA, mint (previous step) and UpboundTOC as the CA loop leaves them. */
+void c2o_toc_step(const c2o_model *m1, const c2o_model *m2,
+                  const c2o_motion *mo1, const c2o_motion *mo2,
+                  const double R1[3][3], const double T1[3],
+                  const double R2[3][3], const double T2[3],
+                  c2o_model *upd1, c2o_model *upd2, c2o_result *res);
+
+/* C2A_QueryTimeOfContact (C2A.cpp:1987-2146).  Seeds the first-step bound
+ * with (res->last_triA, res->last_triB).  Updates m*->last_tri like the
+ * reference.  Returns toc. */
+double c2o_query_toc(c2o_model *m1, c2o_model *m2, c2o_motion *mo1,
+                     c2o_motion *mo2, c2o_result *res, double tol_d,
+                     double tol_t, int max_iter);
+
+/* Contact enumeration at absolute poses (C2A_QueryContact,
+ * C2A.cpp:1937-1966 -> 1727-1775 -> 1525-1725).  Returns the count; fills
+ * up to cap records in reverse DFS order (the reference push_fronts). */
+int c2o_query_contact(const c2o_model *m1, const c2o_model *m2,
+                      const double R1[3][3], const double T1[3],
+                      const double R2[3][3], const double T2[3],
+                      double threshold, c2o_contact *out, int cap,
+                      long long *bv_tests);
+
+/* C2A_Solve (C2A.cpp:2315-2444): tol_d = tol_t = 1e-4, contact threshold
+ * 2d+0.001.  Seeds from m*->last_tri like the demo (mainTorusknot.cpp:314). */
+int c2o_solve(const double R00[3][3], const double T00[3],
+              const double R01[3][3], const double T01[3], c2o_model *m1,
+              const double R10[3][3], const double T10[3],
+              const double R11[3][3], const double T11[3], c2o_model *m2,
+              double R0out[3][3], double T0out[3],
+              double R1out[3][3], double T1out[3], c2o_result *res);
+
+/* Batch helper for tests/baseline: runs nq independent queries with
+ * C2A_QueryTimeOfContact semantics; pose arrays are nq x 12 doubles
+ * (row-major R then T); seeds nq x 2 ints (-1 = use model last_tri).
+ * chain != 0 carries last_tri across queries like the demo.  nthreads > 1
+ * runs queries in parallel (chain must be 0). */
+int c2o_batch_toc(const c2o_model *m1, const c2o_model *m2, int nq,
+                  const double *pose00, const double *pose01,
+                  const double *pose10, const double *pose11,
+                  const int *seeds, double tol_d, double tol_t, int max_iter,
+                  int with_contacts, int chain, int nthreads, c2o_result *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,77 @@
+// ccd_emu.cpp -- TEST HARNESS: runs the GPU kernel's per-group logic
+// (c2a_amd/csrc/ccd_core.h) on the CPU, one group and one lane at a time, so
+// the walk / speculation logic can be checked against the oracle without a
+// GPU.  The product library never links this file.
+#include <string.h>
+
+#include <memory>
+#include <vector>
+
+#include "../../c2a_amd/csrc/bvh.h"
+#include "../../c2a_amd/csrc/ccd_core.h"
+
+using namespace c2a;
+
+template <int K>
+static int run(const KArgs& a) {
+  using Grp = Group<K, 96>;
+  std::unique_ptr<Grp> gp(new Grp());
+  Grp& g = *gp;
+  memset(&g, 0, sizeof(Grp));
+  Leader L;
+  memset(&L, 0, sizeof(L));
+  L.state = ST_FETCH;
+  for (;;) {
+    leader_run(g, L, a);
+    if (L.state == ST_DONE) break;
+    if (g.req)
+      for (int gl = 0; gl < Shape<K>::G; gl++) round_lane(g, gl, a);
+  }
+  return 0;
+}
+
+extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris2, int64_t n2,
+                             const c2a_query* q, int64_t nq, const c2a_params* p, c2a_result* out,
+                             int K) {
+  Bvh b1, b2;
+  if (build_bvh(tris1, nullptr, nullptr, n1, &b1) || build_bvh(tris2, nullptr, nullptr, n2, &b2))
+    return 1;
+  MeshDev tab[2];
+  tab[0].nodes = b1.nodes.data(); tab[0].tris = b1.tris.data();
+  tab[0].ntri = (int32_t)n1; tab[0].depth = b1.max_depth;
+  tab[1].nodes = b2.nodes.data(); tab[1].tris = b2.tris.data();
+  tab[1].ntri = (int32_t)n2; tab[1].depth = b2.max_depth;
+  std::vector<QueryDev> hq((size_t)nq);
+  for (int64_t i = 0; i < nq; i++) {
+    for (int k = 0; k < 2; k++) {
+      hq[i].mesh[k] = q[i].mesh[k];
+      hq[i].seed[k] = q[i].seed_tri[k] < 0 ? 0 : q[i].seed_tri[k];
+      double R0[9], T0[3], R1[9], T1[3];
+      memcpy(R0, q[i].pose[2 * k], 72); memcpy(T0, q[i].pose[2 * k] + 9, 24);
+      memcpy(R1, q[i].pose[2 * k + 1], 72); memcpy(T1, q[i].pose[2 * k + 1] + 9, 24);
+      motion_init(&hq[i].mo[k], R0, T0, R1, T1);
+    }
+  }
+  int64_t counter = 0;
+  KArgs a;
+  memset(&a, 0, sizeof(a));
+  a.meshes = tab;
+  a.q = hq.data();
+  a.out = out;
+  a.counter = &counter;
+  a.nq = nq;
+  a.tol_d = p->tol_d;
+  a.tol_t = p->tol_t;
+  a.contact_scale = p->contact_scale;
+  a.contact_offset = p->contact_offset;
+  a.max_iter = p->max_iter;
+  a.with_contacts = p->with_contacts;
+  a.mode = MODE_QUERY;
+  switch (K) {
+    case 1: return run<1>(a);
+    case 2: return run<2>(a);
+    case 3: return run<3>(a);
+    case 4: return run<4>(a);
+    default: return run<5>(a);
+  }
+}
