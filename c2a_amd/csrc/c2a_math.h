@@ -318,6 +318,29 @@ C2A_HD double rect_edge_tail(double P0, double P1, const double* Rab, const doub
   return sqrt(dot3(S, S));
 }
 
+// True when any active lane of the wavefront holds p (host: the lane itself).
+C2A_HD bool wave_any(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __ballot(p) != 0;
+#else
+  return p;
+#endif
+}
+
+// Ordered (lo, hi) of two values the way the reference builds its edge
+// extents: hi/lo of (x, y) plus the matching pair (z, w).
+C2A_HD void order2(double x, double y, double z, double w, double& lo, double& hi, double& zlo,
+                   double& zhi) {
+  if (x < y) { lo = x; hi = y; zlo = z; zhi = w; }
+  else { lo = y; hi = x; zlo = w; zhi = z; }
+}
+
+// The 16 edge-pair cases are tested in the reference's order; the first one
+// whose extent pre-test and two Voronoi tests hold gives the closest points.
+// The loop over cases is uniform across the wavefront (each case's
+// parameters are formed by a wave-uniform switch) and stops once every lane
+// has its case, so lanes working on different pairs do not serialize the
+// sixteen case bodies.  Each case's expressions are the reference's.
 C2A_HD double rect_dist(const double* Rab, const double* Tab, const double* a, const double* b,
                         double* S) {
   const double AB00 = Rab[0], AB01 = Rab[1], AB10 = Rab[3], AB11 = Rab[4];
@@ -327,154 +350,187 @@ C2A_HD double rect_dist(const double* Rab, const double* Tab, const double* a, c
   const double bAB01 = b[1] * AB01, bAB11 = b[1] * AB11;
   double Tba[3];
   mtxv(Tba, Rab, Tab);
-  double t, u;
+  // corner coordinates of A in B's frame and of B in A's frame
   const double ALLx = -Tba[0];
   const double ALUx = ALLx + aAB10;
   const double AULx = ALLx + aAB00;
   const double AUUx = ALUx + aAB00;
-  double LA1lx, LA1ux, UA1lx, UA1ux;
-  if (ALLx < ALUx) { LA1lx = ALLx; LA1ux = ALUx; UA1lx = AULx; UA1ux = AUUx; }
-  else { LA1lx = ALUx; LA1ux = ALLx; UA1lx = AUUx; UA1ux = AULx; }
   const double BLLx = Tab[0];
   const double BLUx = BLLx + bAB01;
   const double BULx = BLLx + bAB00;
   const double BUUx = BLUx + bAB00;
-  double LB1lx, LB1ux, UB1lx, UB1ux;
-  if (BLLx < BLUx) { LB1lx = BLLx; LB1ux = BLUx; UB1lx = BULx; UB1ux = BUUx; }
-  else { LB1lx = BLUx; LB1ux = BLLx; UB1lx = BUUx; UB1ux = BULx; }
-
-  if ((UA1ux > b[0]) && (UB1ux > a[0]) &&
-      ((UA1lx > b[0]) || voronoi(b[1], a[1], AB10, aAB00 - b[0] - Tba[0], AB11, aAB01 - Tba[1],
-                                 -Tab[1] - bAB10)) &&
-      ((UB1lx > a[0]) || voronoi(a[1], b[1], AB01, Tab[0] + bAB00 - a[0], AB11, Tab[1] + bAB10,
-                                 Tba[1] - aAB01))) {
-    seg_params(t, u, a[1], b[1], AB11, Tab[1] + bAB10, Tba[1] - aAB01);
-    return rect_edge_tail(a[0], t, Rab, Tab, 0, b[0], true, 1, u, S);
-  }
-  if ((UA1lx < 0) && (LB1ux > a[0]) &&
-      ((UA1ux < 0) || voronoi(b[1], a[1], -AB10, Tba[0] - aAB00, AB11, aAB01 - Tba[1], -Tab[1])) &&
-      ((LB1lx > a[0]) || voronoi(a[1], b[1], AB01, Tab[0] - a[0], AB11, Tab[1], Tba[1] - aAB01))) {
-    seg_params(t, u, a[1], b[1], AB11, Tab[1], Tba[1] - aAB01);
-    return rect_edge_tail(a[0], t, Rab, Tab, 0, 0.0, false, 1, u, S);
-  }
-  if ((LA1ux > b[0]) && (UB1lx < 0) &&
-      ((LA1lx > b[0]) || voronoi(b[1], a[1], AB10, -Tba[0] - b[0], AB11, -Tba[1], -Tab[1] - bAB10)) &&
-      ((UB1ux < 0) || voronoi(a[1], b[1], -AB01, -Tab[0] - bAB00, AB11, Tab[1] + bAB10, Tba[1]))) {
-    seg_params(t, u, a[1], b[1], AB11, Tab[1] + bAB10, Tba[1]);
-    return rect_edge_tail(0, t, Rab, Tab, 0, b[0], true, 1, u, S);
-  }
-  if ((LA1lx < 0) && (LB1lx < 0) &&
-      ((LA1ux < 0) || voronoi(b[1], a[1], -AB10, Tba[0], AB11, -Tba[1], -Tab[1])) &&
-      ((LB1ux < 0) || voronoi(a[1], b[1], -AB01, -Tab[0], AB11, Tab[1], Tba[1]))) {
-    seg_params(t, u, a[1], b[1], AB11, Tab[1], Tba[1]);
-    return rect_edge_tail(0, t, Rab, Tab, 0, 0.0, false, 1, u, S);
-  }
-
   const double ALLy = -Tba[1];
   const double ALUy = ALLy + aAB11;
   const double AULy = ALLy + aAB01;
   const double AUUy = ALUy + aAB01;
-  double LA1ly, LA1uy, UA1ly, UA1uy, LB0lx, LB0ux, UB0lx, UB0ux;
-  if (ALLy < ALUy) { LA1ly = ALLy; LA1uy = ALUy; UA1ly = AULy; UA1uy = AUUy; }
-  else { LA1ly = ALUy; LA1uy = ALLy; UA1ly = AUUy; UA1uy = AULy; }
-  if (BLLx < BULx) { LB0lx = BLLx; LB0ux = BULx; UB0lx = BLUx; UB0ux = BUUx; }
-  else { LB0lx = BULx; LB0ux = BLLx; UB0lx = BUUx; UB0ux = BLUx; }
-
-  if ((UA1uy > b[1]) && (UB0ux > a[0]) &&
-      ((UA1ly > b[1]) || voronoi(b[0], a[1], AB11, aAB01 - Tba[1] - b[1], AB10, aAB00 - Tba[0],
-                                 -Tab[1] - bAB11)) &&
-      ((UB0lx > a[0]) || voronoi(a[1], b[0], AB00, Tab[0] - a[0] + bAB01, AB10, Tab[1] + bAB11,
-                                 Tba[0] - aAB00))) {
-    seg_params(t, u, a[1], b[0], AB10, Tab[1] + bAB11, Tba[0] - aAB00);
-    return rect_edge_tail(a[0], t, Rab, Tab, 1, b[1], true, 0, u, S);
-  }
-  if ((UA1ly < 0) && (LB0ux > a[0]) &&
-      ((UA1uy < 0) || voronoi(b[0], a[1], -AB11, Tba[1] - aAB01, AB10, aAB00 - Tba[0], -Tab[1])) &&
-      ((LB0lx > a[0]) || voronoi(a[1], b[0], AB00, Tab[0] - a[0], AB10, Tab[1], Tba[0] - aAB00))) {
-    seg_params(t, u, a[1], b[0], AB10, Tab[1], Tba[0] - aAB00);
-    return rect_edge_tail(a[0], t, Rab, Tab, 1, 0.0, false, 0, u, S);
-  }
-  if ((LA1uy > b[1]) && (UB0lx < 0) &&
-      ((LA1ly > b[1]) || voronoi(b[0], a[1], AB11, -Tba[1] - b[1], AB10, -Tba[0], -Tab[1] - bAB11)) &&
-      ((UB0ux < 0) || voronoi(a[1], b[0], -AB00, -Tab[0] - bAB01, AB10, Tab[1] + bAB11, Tba[0]))) {
-    seg_params(t, u, a[1], b[0], AB10, Tab[1] + bAB11, Tba[0]);
-    return rect_edge_tail(0, t, Rab, Tab, 1, b[1], true, 0, u, S);
-  }
-  if ((LA1ly < 0) && (LB0lx < 0) &&
-      ((LA1uy < 0) || voronoi(b[0], a[1], -AB11, Tba[1], AB10, -Tba[0], -Tab[1])) &&
-      ((LB0ux < 0) || voronoi(a[1], b[0], -AB00, -Tab[0], AB10, Tab[1], Tba[0]))) {
-    seg_params(t, u, a[1], b[0], AB10, Tab[1], Tba[0]);
-    return rect_edge_tail(0, t, Rab, Tab, 1, 0.0, false, 0, u, S);
-  }
-
   const double BLLy = Tab[1];
   const double BLUy = BLLy + bAB11;
   const double BULy = BLLy + bAB10;
   const double BUUy = BLUy + bAB10;
-  double LA0lx, LA0ux, UA0lx, UA0ux, LB1ly, LB1uy, UB1ly, UB1uy;
-  if (ALLx < AULx) { LA0lx = ALLx; LA0ux = AULx; UA0lx = ALUx; UA0ux = AUUx; }
-  else { LA0lx = AULx; LA0ux = ALLx; UA0lx = AUUx; UA0ux = ALUx; }
-  if (BLLy < BLUy) { LB1ly = BLLy; LB1uy = BLUy; UB1ly = BULy; UB1uy = BUUy; }
-  else { LB1ly = BLUy; LB1uy = BLLy; UB1ly = BUUy; UB1uy = BULy; }
 
-  if ((UA0ux > b[0]) && (UB1uy > a[1]) &&
-      ((UA0lx > b[0]) || voronoi(b[1], a[0], AB00, aAB10 - Tba[0] - b[0], AB01, aAB11 - Tba[1],
-                                 -Tab[0] - bAB00)) &&
-      ((UB1ly > a[1]) || voronoi(a[0], b[1], AB11, Tab[1] - a[1] + bAB10, AB01, Tab[0] + bAB00,
-                                 Tba[1] - aAB11))) {
-    seg_params(t, u, a[0], b[1], AB01, Tab[0] + bAB00, Tba[1] - aAB11);
-    return rect_edge_tail(t, a[1], Rab, Tab, 0, b[0], true, 1, u, S);
+  int win = -1;
+  double g_a = 0, g_b = 0, g_AB = 0, g_AT = 0, g_BT = 0;   // segment parameters
+  double g_p = 0;                                          // fixed coordinate of P
+  int g_tx = 0;                                            // 1: P = (t, p), 0: P = (p, t)
+  int g_cf = 0, g_cv = 0;
+  double g_lf = 0;
+  bool g_hf = false;
+  for (int c = 0; c < 16; c++) {
+    if (!wave_any(win < 0)) break;
+    double lo1, hi1, lo2, hi2, ulo1, uhi1, ulo2, uhi2;
+    bool pre = false, s1 = false, s2 = false;
+    double v1[7], v2[7], sg[5];
+    double pfix = 0;
+    int tx = 0, cf = 0, cv = 0;
+    double lf = 0;
+    bool hf = false;
+    if (c < 4) {
+      order2(ALLx, ALUx, AULx, AUUx, lo1, hi1, ulo1, uhi1);   // LA1 x-extent, UA1 x-extent
+      order2(BLLx, BLUx, BULx, BUUx, lo2, hi2, ulo2, uhi2);   // LB1, UB1
+    } else if (c < 8) {
+      order2(ALLy, ALUy, AULy, AUUy, lo1, hi1, ulo1, uhi1);   // LA1 y, UA1 y
+      order2(BLLx, BULx, BLUx, BUUx, lo2, hi2, ulo2, uhi2);   // LB0 x, UB0 x
+    } else if (c < 12) {
+      order2(ALLx, AULx, ALUx, AUUx, lo1, hi1, ulo1, uhi1);   // LA0 x, UA0 x
+      order2(BLLy, BLUy, BULy, BUUy, lo2, hi2, ulo2, uhi2);   // LB1 y, UB1 y
+    } else {
+      order2(ALLy, AULy, ALUy, AUUy, lo1, hi1, ulo1, uhi1);   // LA0 y, UA0 y
+      order2(BLLy, BULy, BLUy, BUUy, lo2, hi2, ulo2, uhi2);   // LB0 y, UB0 y
+    }
+#define C2A_V(o, A0, A1, A2, A3, A4, A5, A6) \
+  o[0] = (A0); o[1] = (A1); o[2] = (A2); o[3] = (A3); o[4] = (A4); o[5] = (A5); o[6] = (A6)
+#define C2A_G(A0, A1, A2, A3, A4) sg[0] = (A0); sg[1] = (A1); sg[2] = (A2); sg[3] = (A3); sg[4] = (A4)
+    switch (c) {
+      case 0:   // A x=a0 edge, B b0 edge (both along the 1-axes)
+        pre = (uhi1 > b[0]) && (uhi2 > a[0]); s1 = (ulo1 > b[0]); s2 = (ulo2 > a[0]);
+        C2A_V(v1, b[1], a[1], AB10, aAB00 - b[0] - Tba[0], AB11, aAB01 - Tba[1], -Tab[1] - bAB10);
+        C2A_V(v2, a[1], b[1], AB01, Tab[0] + bAB00 - a[0], AB11, Tab[1] + bAB10, Tba[1] - aAB01);
+        C2A_G(a[1], b[1], AB11, Tab[1] + bAB10, Tba[1] - aAB01);
+        pfix = a[0]; tx = 0; cf = 0; lf = b[0]; hf = true; cv = 1;
+        break;
+      case 1:
+        pre = (ulo1 < 0) && (hi2 > a[0]); s1 = (uhi1 < 0); s2 = (lo2 > a[0]);
+        C2A_V(v1, b[1], a[1], -AB10, Tba[0] - aAB00, AB11, aAB01 - Tba[1], -Tab[1]);
+        C2A_V(v2, a[1], b[1], AB01, Tab[0] - a[0], AB11, Tab[1], Tba[1] - aAB01);
+        C2A_G(a[1], b[1], AB11, Tab[1], Tba[1] - aAB01);
+        pfix = a[0]; tx = 0; cv = 1;
+        break;
+      case 2:
+        pre = (hi1 > b[0]) && (ulo2 < 0); s1 = (lo1 > b[0]); s2 = (uhi2 < 0);
+        C2A_V(v1, b[1], a[1], AB10, -Tba[0] - b[0], AB11, -Tba[1], -Tab[1] - bAB10);
+        C2A_V(v2, a[1], b[1], -AB01, -Tab[0] - bAB00, AB11, Tab[1] + bAB10, Tba[1]);
+        C2A_G(a[1], b[1], AB11, Tab[1] + bAB10, Tba[1]);
+        pfix = 0; tx = 0; cf = 0; lf = b[0]; hf = true; cv = 1;
+        break;
+      case 3:
+        pre = (lo1 < 0) && (lo2 < 0); s1 = (hi1 < 0); s2 = (hi2 < 0);
+        C2A_V(v1, b[1], a[1], -AB10, Tba[0], AB11, -Tba[1], -Tab[1]);
+        C2A_V(v2, a[1], b[1], -AB01, -Tab[0], AB11, Tab[1], Tba[1]);
+        C2A_G(a[1], b[1], AB11, Tab[1], Tba[1]);
+        pfix = 0; tx = 0; cv = 1;
+        break;
+      case 4:   // A 1-axis edges against B 0-axis edges
+        pre = (uhi1 > b[1]) && (uhi2 > a[0]); s1 = (ulo1 > b[1]); s2 = (ulo2 > a[0]);
+        C2A_V(v1, b[0], a[1], AB11, aAB01 - Tba[1] - b[1], AB10, aAB00 - Tba[0], -Tab[1] - bAB11);
+        C2A_V(v2, a[1], b[0], AB00, Tab[0] - a[0] + bAB01, AB10, Tab[1] + bAB11, Tba[0] - aAB00);
+        C2A_G(a[1], b[0], AB10, Tab[1] + bAB11, Tba[0] - aAB00);
+        pfix = a[0]; tx = 0; cf = 1; lf = b[1]; hf = true; cv = 0;
+        break;
+      case 5:
+        pre = (ulo1 < 0) && (hi2 > a[0]); s1 = (uhi1 < 0); s2 = (lo2 > a[0]);
+        C2A_V(v1, b[0], a[1], -AB11, Tba[1] - aAB01, AB10, aAB00 - Tba[0], -Tab[1]);
+        C2A_V(v2, a[1], b[0], AB00, Tab[0] - a[0], AB10, Tab[1], Tba[0] - aAB00);
+        C2A_G(a[1], b[0], AB10, Tab[1], Tba[0] - aAB00);
+        pfix = a[0]; tx = 0; cv = 0;
+        break;
+      case 6:
+        pre = (hi1 > b[1]) && (ulo2 < 0); s1 = (lo1 > b[1]); s2 = (uhi2 < 0);
+        C2A_V(v1, b[0], a[1], AB11, -Tba[1] - b[1], AB10, -Tba[0], -Tab[1] - bAB11);
+        C2A_V(v2, a[1], b[0], -AB00, -Tab[0] - bAB01, AB10, Tab[1] + bAB11, Tba[0]);
+        C2A_G(a[1], b[0], AB10, Tab[1] + bAB11, Tba[0]);
+        pfix = 0; tx = 0; cf = 1; lf = b[1]; hf = true; cv = 0;
+        break;
+      case 7:
+        pre = (lo1 < 0) && (lo2 < 0); s1 = (hi1 < 0); s2 = (hi2 < 0);
+        C2A_V(v1, b[0], a[1], -AB11, Tba[1], AB10, -Tba[0], -Tab[1]);
+        C2A_V(v2, a[1], b[0], -AB00, -Tab[0], AB10, Tab[1], Tba[0]);
+        C2A_G(a[1], b[0], AB10, Tab[1], Tba[0]);
+        pfix = 0; tx = 0; cv = 0;
+        break;
+      case 8:   // A 0-axis edges against B 1-axis edges
+        pre = (uhi1 > b[0]) && (uhi2 > a[1]); s1 = (ulo1 > b[0]); s2 = (ulo2 > a[1]);
+        C2A_V(v1, b[1], a[0], AB00, aAB10 - Tba[0] - b[0], AB01, aAB11 - Tba[1], -Tab[0] - bAB00);
+        C2A_V(v2, a[0], b[1], AB11, Tab[1] - a[1] + bAB10, AB01, Tab[0] + bAB00, Tba[1] - aAB11);
+        C2A_G(a[0], b[1], AB01, Tab[0] + bAB00, Tba[1] - aAB11);
+        pfix = a[1]; tx = 1; cf = 0; lf = b[0]; hf = true; cv = 1;
+        break;
+      case 9:
+        pre = (ulo1 < 0) && (hi2 > a[1]); s1 = (uhi1 < 0); s2 = (lo2 > a[1]);
+        C2A_V(v1, b[1], a[0], -AB00, Tba[0] - aAB10, AB01, aAB11 - Tba[1], -Tab[0]);
+        C2A_V(v2, a[0], b[1], AB11, Tab[1] - a[1], AB01, Tab[0], Tba[1] - aAB11);
+        C2A_G(a[0], b[1], AB01, Tab[0], Tba[1] - aAB11);
+        pfix = a[1]; tx = 1; cv = 1;
+        break;
+      case 10:
+        pre = (hi1 > b[0]) && (ulo2 < 0); s1 = (lo1 > b[0]); s2 = (uhi2 < 0);
+        C2A_V(v1, b[1], a[0], AB00, -b[0] - Tba[0], AB01, -Tba[1], -bAB00 - Tab[0]);
+        C2A_V(v2, a[0], b[1], -AB11, -Tab[1] - bAB10, AB01, Tab[0] + bAB00, Tba[1]);
+        C2A_G(a[0], b[1], AB01, Tab[0] + bAB00, Tba[1]);
+        pfix = 0; tx = 1; cf = 0; lf = b[0]; hf = true; cv = 1;
+        break;
+      case 11:
+        pre = (lo1 < 0) && (lo2 < 0); s1 = (hi1 < 0); s2 = (hi2 < 0);
+        C2A_V(v1, b[1], a[0], -AB00, Tba[0], AB01, -Tba[1], -Tab[0]);
+        C2A_V(v2, a[0], b[1], -AB11, -Tab[1], AB01, Tab[0], Tba[1]);
+        C2A_G(a[0], b[1], AB01, Tab[0], Tba[1]);
+        pfix = 0; tx = 1; cv = 1;
+        break;
+      case 12:  // A 0-axis edges against B 0-axis edges
+        pre = (uhi1 > b[1]) && (uhi2 > a[1]); s1 = (ulo1 > b[1]); s2 = (ulo2 > a[1]);
+        C2A_V(v1, b[0], a[0], AB01, aAB11 - Tba[1] - b[1], AB00, aAB10 - Tba[0], -Tab[0] - bAB01);
+        C2A_V(v2, a[0], b[0], AB10, Tab[1] - a[1] + bAB11, AB00, Tab[0] + bAB01, Tba[0] - aAB10);
+        C2A_G(a[0], b[0], AB00, Tab[0] + bAB01, Tba[0] - aAB10);
+        pfix = a[1]; tx = 1; cf = 1; lf = b[1]; hf = true; cv = 0;
+        break;
+      case 13:
+        pre = (ulo1 < 0) && (hi2 > a[1]); s1 = (uhi1 < 0); s2 = (lo2 > a[1]);
+        C2A_V(v1, b[0], a[0], -AB01, Tba[1] - aAB11, AB00, aAB10 - Tba[0], -Tab[0]);
+        C2A_V(v2, a[0], b[0], AB10, Tab[1] - a[1], AB00, Tab[0], Tba[0] - aAB10);
+        C2A_G(a[0], b[0], AB00, Tab[0], Tba[0] - aAB10);
+        pfix = a[1]; tx = 1; cv = 0;
+        break;
+      case 14:
+        pre = (hi1 > b[1]) && (ulo2 < 0); s1 = (lo1 > b[1]); s2 = (uhi2 < 0);
+        C2A_V(v1, b[0], a[0], AB01, -Tba[1] - b[1], AB00, -Tba[0], -Tab[0] - bAB01);
+        C2A_V(v2, a[0], b[0], -AB10, -Tab[1] - bAB11, AB00, Tab[0] + bAB01, Tba[0]);
+        C2A_G(a[0], b[0], AB00, Tab[0] + bAB01, Tba[0]);
+        pfix = 0; tx = 1; cf = 1; lf = b[1]; hf = true; cv = 0;
+        break;
+      default:
+        pre = (lo1 < 0) && (lo2 < 0); s1 = (hi1 < 0); s2 = (hi2 < 0);
+        C2A_V(v1, b[0], a[0], -AB01, Tba[1], AB00, -Tba[0], -Tab[0]);
+        C2A_V(v2, a[0], b[0], -AB10, -Tab[1], AB00, Tab[0], Tba[0]);
+        C2A_G(a[0], b[0], AB00, Tab[0], Tba[0]);
+        pfix = 0; tx = 1; cv = 0;
+        break;
+    }
+#undef C2A_V
+#undef C2A_G
+    if (win < 0 && pre) {
+      bool ok = s1 || voronoi(v1[0], v1[1], v1[2], v1[3], v1[4], v1[5], v1[6]);
+      ok = ok && (s2 || voronoi(v2[0], v2[1], v2[2], v2[3], v2[4], v2[5], v2[6]));
+      if (ok) {
+        win = c;
+        g_a = sg[0]; g_b = sg[1]; g_AB = sg[2]; g_AT = sg[3]; g_BT = sg[4];
+        g_p = pfix; g_tx = tx; g_cf = cf; g_cv = cv; g_lf = lf; g_hf = hf;
+      }
+    }
   }
-  if ((UA0lx < 0) && (LB1uy > a[1]) &&
-      ((UA0ux < 0) || voronoi(b[1], a[0], -AB00, Tba[0] - aAB10, AB01, aAB11 - Tba[1], -Tab[0])) &&
-      ((LB1ly > a[1]) || voronoi(a[0], b[1], AB11, Tab[1] - a[1], AB01, Tab[0], Tba[1] - aAB11))) {
-    seg_params(t, u, a[0], b[1], AB01, Tab[0], Tba[1] - aAB11);
-    return rect_edge_tail(t, a[1], Rab, Tab, 0, 0.0, false, 1, u, S);
-  }
-  if ((LA0ux > b[0]) && (UB1ly < 0) &&
-      ((LA0lx > b[0]) || voronoi(b[1], a[0], AB00, -b[0] - Tba[0], AB01, -Tba[1], -bAB00 - Tab[0])) &&
-      ((UB1uy < 0) || voronoi(a[0], b[1], -AB11, -Tab[1] - bAB10, AB01, Tab[0] + bAB00, Tba[1]))) {
-    seg_params(t, u, a[0], b[1], AB01, Tab[0] + bAB00, Tba[1]);
-    return rect_edge_tail(t, 0, Rab, Tab, 0, b[0], true, 1, u, S);
-  }
-  if ((LA0lx < 0) && (LB1ly < 0) &&
-      ((LA0ux < 0) || voronoi(b[1], a[0], -AB00, Tba[0], AB01, -Tba[1], -Tab[0])) &&
-      ((LB1uy < 0) || voronoi(a[0], b[1], -AB11, -Tab[1], AB01, Tab[0], Tba[1]))) {
-    seg_params(t, u, a[0], b[1], AB01, Tab[0], Tba[1]);
-    return rect_edge_tail(t, 0, Rab, Tab, 0, 0.0, false, 1, u, S);
-  }
-
-  double LA0ly, LA0uy, UA0ly, UA0uy, LB0ly, LB0uy, UB0ly, UB0uy;
-  if (ALLy < AULy) { LA0ly = ALLy; LA0uy = AULy; UA0ly = ALUy; UA0uy = AUUy; }
-  else { LA0ly = AULy; LA0uy = ALLy; UA0ly = AUUy; UA0uy = ALUy; }
-  if (BLLy < BULy) { LB0ly = BLLy; LB0uy = BULy; UB0ly = BLUy; UB0uy = BUUy; }
-  else { LB0ly = BULy; LB0uy = BLLy; UB0ly = BUUy; UB0uy = BLUy; }
-
-  if ((UA0uy > b[1]) && (UB0uy > a[1]) &&
-      ((UA0ly > b[1]) || voronoi(b[0], a[0], AB01, aAB11 - Tba[1] - b[1], AB00, aAB10 - Tba[0],
-                                 -Tab[0] - bAB01)) &&
-      ((UB0ly > a[1]) || voronoi(a[0], b[0], AB10, Tab[1] - a[1] + bAB11, AB00, Tab[0] + bAB01,
-                                 Tba[0] - aAB10))) {
-    seg_params(t, u, a[0], b[0], AB00, Tab[0] + bAB01, Tba[0] - aAB10);
-    return rect_edge_tail(t, a[1], Rab, Tab, 1, b[1], true, 0, u, S);
-  }
-  if ((UA0ly < 0) && (LB0uy > a[1]) &&
-      ((UA0uy < 0) || voronoi(b[0], a[0], -AB01, Tba[1] - aAB11, AB00, aAB10 - Tba[0], -Tab[0])) &&
-      ((LB0ly > a[1]) || voronoi(a[0], b[0], AB10, Tab[1] - a[1], AB00, Tab[0], Tba[0] - aAB10))) {
-    seg_params(t, u, a[0], b[0], AB00, Tab[0], Tba[0] - aAB10);
-    return rect_edge_tail(t, a[1], Rab, Tab, 1, 0.0, false, 0, u, S);
-  }
-  if ((LA0uy > b[1]) && (UB0ly < 0) &&
-      ((LA0ly > b[1]) || voronoi(b[0], a[0], AB01, -Tba[1] - b[1], AB00, -Tba[0], -Tab[0] - bAB01)) &&
-      ((UB0uy < 0) || voronoi(a[0], b[0], -AB10, -Tab[1] - bAB11, AB00, Tab[0] + bAB01, Tba[0]))) {
-    seg_params(t, u, a[0], b[0], AB00, Tab[0] + bAB01, Tba[0]);
-    return rect_edge_tail(t, 0, Rab, Tab, 1, b[1], true, 0, u, S);
-  }
-  if ((LA0ly < 0) && (LB0ly < 0) &&
-      ((LA0uy < 0) || voronoi(b[0], a[0], -AB01, Tba[1], AB00, -Tba[0], -Tab[0])) &&
-      ((LB0uy < 0) || voronoi(a[0], b[0], -AB10, -Tab[1], AB00, Tab[0], Tba[0]))) {
-    seg_params(t, u, a[0], b[0], AB00, Tab[0], Tba[0]);
-    return rect_edge_tail(t, 0, Rab, Tab, 1, 0.0, false, 0, u, S);
+  if (win >= 0) {
+    double t, u;
+    seg_params(t, u, g_a, g_b, g_AB, g_AT, g_BT);
+    return g_tx ? rect_edge_tail(t, g_p, Rab, Tab, g_cf, g_lf, g_hf, g_cv, u, S)
+                : rect_edge_tail(g_p, t, Rab, Tab, g_cf, g_lf, g_hf, g_cv, u, S);
   }
 
   // face-normal separation (C2A_RectDist.h:855-933)

@@ -84,7 +84,7 @@ struct c2a_batch {
   DevBuf d_mesh, d_q, d_out, d_counter;
   int64_t nq = 0;
   KArgs args{};
-  int K = 3, NG = 4, CAP = 48, blocks = 1;
+  int K = 5, CAP = 48, blocks = 1;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
   ~c2a_batch() {
@@ -134,16 +134,13 @@ int pick_cap(const std::vector<MeshDev>& tab, const int32_t* pairs, int64_t n, i
   return -1;
 }
 
-int pick_shape(const c2a_params* p, int64_t nq, int& K, int& NG) {
-  K = (p && p->spec_depth > 0) ? p->spec_depth : 3;
-  if (K < 1 || K > 5) return fail(C2A_ERR_BAD_ARG, "spec_depth must be 1..5");
-  int maxg = kernel_groups(K);
-  NG = (K <= 2) ? 8 : maxg;
-  (void)nq;
+int pick_shape(const c2a_params* p, int& K) {
+  K = (p && p->spec_depth > 0) ? p->spec_depth : 5;
+  if (K < 1 || K > 6) return fail(C2A_ERR_BAD_ARG, "spec_depth must be 1..6");
   return C2A_OK;
 }
 
-int grid_blocks(int K, int NG, int CAP, int64_t nq) {
+int grid_blocks(int K, int CAP, int64_t nq) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   hipDeviceProp_t prop;
@@ -153,10 +150,10 @@ int grid_blocks(int K, int NG, int CAP, int64_t nq) {
     cus = prop.multiProcessorCount;
     if (prop.maxSharedMemoryPerMultiProcessor > 0) lds_cu = prop.maxSharedMemoryPerMultiProcessor;
   }
-  size_t lds = kernel_lds_bytes(K, NG, CAP);
+  size_t lds = kernel_lds_bytes(K, CAP);
   int per_cu = lds ? (int)std::max<size_t>(1, lds_cu / lds) : 1;
   per_cu = std::min(per_cu, 16);
-  int64_t want = (nq + NG - 1) / NG;
+  int64_t want = nq;
   int64_t cap = (int64_t)cus * per_cu;
   return (int)std::max<int64_t>(1, std::min(want, cap));
 }
@@ -208,11 +205,11 @@ int fill_args(c2a_batch* b, const c2a_params* p) {
 
 __global__ void zero_counter(int64_t* c) { *c = 0; }
 
-int run_kernel(const KArgs& a, int K, int NG, int CAP, int blocks, hipStream_t s) {
+int run_kernel(const KArgs& a, int K, int CAP, int blocks, hipStream_t s) {
   // counter cleared by a kernel (graph-capture safe, no memset node)
   hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, a.counter);
   HIP_TRY(hipGetLastError());
-  hipError_t e = launch_ccd(a, K, NG, CAP, blocks, s);
+  hipError_t e = launch_ccd(a, K, CAP, blocks, s);
   if (e != hipSuccess) return hip_fail(e, "launch_ccd");
   return C2A_OK;
 }
@@ -321,7 +318,7 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     return fail(C2A_ERR_OOM, "host allocation failed");
   }
   b->nq = nq;
-  if ((rc = pick_shape(params, nq, b->K, b->NG))) {
+  if ((rc = pick_shape(params, b->K))) {
     delete b;
     return rc;
   }
@@ -330,7 +327,7 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     delete b;
     return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
   }
-  b->blocks = grid_blocks(b->K, b->NG, b->CAP, nq);
+  b->blocks = grid_blocks(b->K, b->CAP, nq);
   if ((rc = b->d_mesh.alloc(b->mtab.size() * sizeof(MeshDev))) ||
       (rc = b->d_q.alloc((size_t)std::max<int64_t>(nq, 1) * sizeof(QueryDev))) ||
       (rc = b->d_out.alloc((size_t)std::max<int64_t>(nq, 1) * sizeof(c2a_result))) ||
@@ -358,7 +355,7 @@ int c2a_batch_run(c2a_batch* b, void* stream) {
   if (b->nq == 0) return C2A_OK;
   hipStream_t s = (hipStream_t)stream;
   HIP_TRY(hipEventRecord(b->ev0, s));
-  int rc = run_kernel(b->args, b->K, b->NG, b->CAP, b->blocks, s);
+  int rc = run_kernel(b->args, b->K, b->CAP, b->blocks, s);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(b->ev1, s));
   b->timed = true;
@@ -438,7 +435,7 @@ int c2a_ccd_step(c2a_mesh* const* meshes, int32_t nmesh, const c2a_step_query* q
   }
   int CAP = pick_cap(tab, pairs.data(), nq, 2);
   if (CAP < 0) return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
-  int K = 3, NG = kernel_groups(3);
+  const int K = 5;
   DevBuf dm, din, dout, dc;
   if ((rc = dm.alloc(tab.size() * sizeof(MeshDev))) || (rc = din.alloc((size_t)nq * sizeof(StepDev))) ||
       (rc = dout.alloc((size_t)nq * sizeof(c2a_step_result))) || (rc = dc.alloc(sizeof(int64_t))))
@@ -455,7 +452,7 @@ int c2a_ccd_step(c2a_mesh* const* meshes, int32_t nmesh, const c2a_step_query* q
   a.nq = nq;
   a.mode = MODE_STEP;
   a.max_iter = 150;
-  if ((rc = run_kernel(a, K, NG, CAP, grid_blocks(K, NG, CAP, nq), s))) return rc;
+  if ((rc = run_kernel(a, K, CAP, grid_blocks(K, CAP, nq), s))) return rc;
   HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)nq * sizeof(c2a_step_result), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return C2A_OK;
@@ -486,7 +483,7 @@ int c2a_contact_count(c2a_mesh* const* meshes, int32_t nmesh, const double* pose
   }
   int CAP = pick_cap(tab, mesh_pairs, nq, 2);
   if (CAP < 0) return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
-  int K = 3, NG = kernel_groups(3);
+  const int K = 5;
   DevBuf dm, din, dout, dc;
   if ((rc = dm.alloc(tab.size() * sizeof(MeshDev))) ||
       (rc = din.alloc((size_t)nq * sizeof(ContactDev))) ||
@@ -503,7 +500,7 @@ int c2a_contact_count(c2a_mesh* const* meshes, int32_t nmesh, const double* pose
   a.counter = (int64_t*)dc.p;
   a.nq = nq;
   a.mode = MODE_CONTACT;
-  if ((rc = run_kernel(a, K, NG, CAP, grid_blocks(K, NG, CAP, nq), s))) return rc;
+  if ((rc = run_kernel(a, K, CAP, grid_blocks(K, CAP, nq), s))) return rc;
   HIP_TRY(hipMemcpyAsync(counts_out, dout.p, (size_t)nq * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return C2A_OK;

@@ -7,10 +7,8 @@
 
 namespace c2a {
 
-// K = speculative subtree depth, NG = query groups per wavefront,
-// CAP = DFS stack capacity (entries) per group.
-hipError_t launch_ccd(const KArgs& a, int K, int NG, int CAP, int blocks, hipStream_t s);
-size_t kernel_lds_bytes(int K, int NG, int CAP);
-int kernel_groups(int K);
+// K = speculative subtree depth (1..6), CAP = DFS stack capacity in entries.
+hipError_t launch_ccd(const KArgs& a, int K, int CAP, int blocks, hipStream_t s);
+size_t kernel_lds_bytes(int K, int CAP);
 
 }  // namespace c2a

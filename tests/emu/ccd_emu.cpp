@@ -14,18 +14,17 @@ using namespace c2a;
 
 template <int K>
 static int run(const KArgs& a) {
-  using Grp = Group<K, 96>;
-  std::unique_ptr<Grp> gp(new Grp());
-  Grp& g = *gp;
-  memset(&g, 0, sizeof(Grp));
+  using W = Wave<K, 96>;
+  std::unique_ptr<W> gp(new W());
+  W& g = *gp;
+  memset(&g, 0, sizeof(W));
   Leader L;
   memset(&L, 0, sizeof(L));
   L.state = ST_FETCH;
   for (;;) {
-    leader_run(g, L, a);
+    leader_run(g, L, a, 0, 1);
     if (L.state == ST_DONE) break;
-    if (g.req)
-      for (int gl = 0; gl < Shape<K>::G; gl++) round_lane(g, gl, a);
+    round_lane(g, 0, 1, a);
   }
   return 0;
 }
@@ -72,6 +71,7 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
     case 2: return run<2>(a);
     case 3: return run<3>(a);
     case 4: return run<4>(a);
-    default: return run<5>(a);
+    case 5: return run<5>(a);
+    default: return run<6>(a);
   }
 }
