@@ -81,10 +81,11 @@ struct c2a_mesh {
 
 struct c2a_batch {
   std::vector<MeshDev> mtab;
-  DevBuf d_mesh, d_q, d_out, d_counter;
+  DevBuf d_mesh, d_q, d_out, d_counter, d_arena;
   int64_t nq = 0;
   KArgs args{};
   int K = 5, CAP = 48, blocks = 1;
+  int logcap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
   ~c2a_batch() {
@@ -200,6 +201,8 @@ int fill_args(c2a_batch* b, const c2a_params* p) {
   a.max_iter = p->max_iter;
   a.with_contacts = p->with_contacts;
   a.mode = MODE_QUERY;
+  a.logcap = b->logcap;
+  a.arena = (unsigned char*)b->d_arena.p;
   return C2A_OK;
 }
 
@@ -229,6 +232,8 @@ void c2a_params_default(c2a_params* p) {
   p->contact_offset = 0.001;
   p->spec_depth = 0;
   p->groups_per_wave = 0;
+  p->replay_log = 8192;
+  p->pad = 0;
 }
 
 int c2a_mesh_create(const double* tri_xyz9, const int32_t* tri_ids, const int32_t* vert_idx3,
@@ -328,7 +333,19 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
   }
   b->blocks = grid_blocks(b->K, b->CAP, nq);
-  if ((rc = b->d_mesh.alloc(b->mtab.size() * sizeof(MeshDev))) ||
+  {
+    c2a_params def;
+    c2a_params_default(&def);
+    const c2a_params* pp = params ? params : &def;
+    if (pp->replay_log < 0 || pp->replay_log > (1 << 24)) {
+      delete b;
+      return fail(C2A_ERR_BAD_ARG, "replay_log must be 0..2^24");
+    }
+    b->logcap = pp->replay_log;
+  }
+  if ((b->logcap > 0 && nq > 0 &&
+       (rc = b->d_arena.alloc((size_t)b->blocks * arena_bytes(b->logcap)))) ||
+      (rc = b->d_mesh.alloc(b->mtab.size() * sizeof(MeshDev))) ||
       (rc = b->d_q.alloc((size_t)std::max<int64_t>(nq, 1) * sizeof(QueryDev))) ||
       (rc = b->d_out.alloc((size_t)std::max<int64_t>(nq, 1) * sizeof(c2a_result))) ||
       (rc = b->d_counter.alloc(sizeof(int64_t)))) {

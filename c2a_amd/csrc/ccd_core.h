@@ -12,6 +12,13 @@
 // directional motion bound and, for leaf pairs, the triangle distance.  Pair
 // values never depend on the traversal state, so speculation cannot change a
 // result; it only lets one round feed up to K levels of the walk.
+//
+// Replay: the walk logs every expansion of a CA step.  At the next step the
+// lanes first recompute the logged subtree at the new poses, one BVTT level
+// at a time (replay), and the walk then takes children from those records;
+// only pairs the previous step did not expand need speculative rounds.  The
+// records hold exactly the values a round would compute for the same pairs,
+// so replay changes the speed of a step and nothing else.
 #pragma once
 
 #include <stdint.h>
@@ -19,6 +26,10 @@
 #include "bvh.h"
 #include "c2a_math.h"
 #include "ccd_types.h"
+
+#ifndef C2A_EMU_COUNT
+#define C2A_EMU_COUNT(kind, L) ((void)0)
+#endif
 
 namespace c2a {
 
@@ -30,10 +41,34 @@ C2A_HD int clz32(unsigned x) {
 #endif
 }
 
-// Makes the LDS writes of all lanes visible to all lanes (one wave per block).
+// Makes the LDS writes of all lanes visible to all lanes.  A workgroup is
+// one wavefront, whose LDS accesses complete in issue order, so this only has
+// to keep the compiler from reordering LDS accesses across it and drain the
+// LDS counter; outstanding global stores are not waited for.
 C2A_HD void lane_sync() {
 #if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#endif
+}
+
+// Makes global-memory stores of all lanes visible to loads by other lanes of
+// the workgroup (the replay passes exchange data through the arena).
+C2A_HD void mem_sync() {
+#if defined(__HIP_DEVICE_COMPILE__)
   __syncthreads();
+#endif
+}
+
+// Fetch-and-add on an LDS counter shared by the lanes of the wavefront.
+C2A_HD int lds_add(int32_t* p, int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicAdd(p, v);
+#else
+  int o = *p;
+  *p = o + v;
+  return o;
 #endif
 }
 
@@ -47,16 +82,28 @@ C2A_HD void lane_sync() {
 typedef const GpuNode C2A_GLOBAL* node_ptr;
 typedef const double C2A_GLOBAL* gdouble_ptr;
 
+template <class T>
+C2A_HD T C2A_GLOBAL* gp(T* p) {
+  return (T C2A_GLOBAL*)p;
+}
+
 C2A_HD node_ptr nodes_of(const MeshDev& m) { return (node_ptr)m.nodes; }
 C2A_HD gdouble_ptr tris_of(const MeshDev& m) { return (gdouble_ptr)m.tris; }
 
-struct Entry {           // 128 B stack entry: a considered-but-undecided pair
-  double R[9], T[3];     // pair transform (model-2 BV in model-1 BV frame)
+struct Entry {           // 160 B stack entry: a considered-but-undecided pair
+  double R[9], T[3];     // pair transform (model-2 BV in model-1 BV frame); for
+                         // replayed entries it lives in the arena instead
   double d, mt;          // RSS distance and admissible time of the pair
-  int32_t b1, b2;        // node indices
+  int32_t b1, b2;        // node indices (replayed leaf pairs: triangle indices)
   int32_t sg;            // (generation << 8) | (slot + 1) of cached values, 0 = none
-  int32_t flags;         // bit0 leaf1, bit1 leaf2, bit2 root (always visited)
+  int32_t flags;         // bit0 leaf1, bit1 leaf2, bit2 root (always visited),
+                         // bit3 replayed record
+  double dTri, mleaf;    // replayed leaf pairs: triangle distance, leaf time
+  int32_t ar, cp;        // replayed: record number, expansion index of its children
+  int32_t nn, lvl;       // record number in this step's log (-1 = unlogged), depth
 };
+
+constexpr int WIN = 64;  // replayed records staged in LDS for the walk
 
 struct Slot {            // speculative pair values, heap-indexed below X
   double R[9], T[3];     // same first 14 words as Entry
@@ -86,6 +133,10 @@ struct Wave {
   int32_t mesh[2];
   int32_t contact;           // 1: contact-enumeration traversal (no motion bounds)
   int32_t gen;
+  HotRec win[WIN];           // replayed records [wbase, wbase + WIN)
+  int32_t wbase;
+  int32_t lh[CAP + 2];       // replay: expansions per level, then fill cursors
+  int32_t ls[CAP + 3];       // replay: first sorted expansion of each level
 };
 
 enum : int { ST_FETCH = 0, ST_STEP = 1, ST_WALK = 2, ST_CONTACT_WALK = 3, ST_DONE = 4 };
@@ -108,18 +159,46 @@ struct Leader {
   int num_contact;
   double thr;
   int mode;
+  int nlog, nlog_prev, logbuf;   // expansions logged this step / by the previous step
+  int pq_pending;                // p1, p2 of the last improving leaf still to compute
+  unsigned char* ab;             // this wavefront's replay arena
 };
 
 C2A_HD int node_leaf(node_ptr n) { return n->first_child < 0; }
 
 // Lane-parallel copy of the 14 value words + node ids of a slot into a stack
-// entry; word 15 (sg, flags) is written by one lane.
-C2A_HD void slot_to_entry(Entry& dst, const Slot& src, int sg, int flags, int lane, int nl) {
+// entry; the remaining words (sg/flags, replay fields) are set per lane.
+C2A_HD void slot_to_entry(Entry& dst, const Slot& src, int sg, int flags, int nn, int lvl,
+                          int lane, int nl) {
   const uint64_t* s = reinterpret_cast<const uint64_t*>(&src);
   uint64_t* d = reinterpret_cast<uint64_t*>(&dst);
-  for (int w = lane; w < 16; w += nl) {
+  for (int w = lane; w < 20; w += nl) {
     if (w < 15) d[w] = s[w];
-    else { dst.sg = sg; dst.flags = flags; }
+    else if (w == 15) { dst.sg = sg; dst.flags = flags; }
+    else if (w == 16) dst.dTri = 0.0;
+    else if (w == 17) dst.mleaf = 0.0;
+    else if (w == 18) { dst.ar = -1; dst.cp = -1; }
+    else { dst.nn = nn; dst.lvl = lvl; }
+  }
+}
+
+// Stack entry for replayed record `rec` (values from the LDS window).  The
+// transform stays in the arena; leaf pairs carry their triangle indices in
+// b1/b2.
+C2A_HD void hot_to_entry(Entry& dst, const HotRec& h, int rec, int nn, int lvl, int lane,
+                         int nl) {
+  const bool leaf = (h.flags & 3) == 3;
+  for (int w = lane; w < 8; w += nl) {
+    switch (w) {
+      case 0: dst.d = h.d; break;
+      case 1: dst.mt = h.mt; break;
+      case 2: dst.b1 = leaf ? h.t1 : h.b1; dst.b2 = leaf ? h.t2 : h.b2; break;
+      case 3: dst.sg = 0; dst.flags = (h.flags & 3) | 8; break;
+      case 4: dst.dTri = h.dTri; break;
+      case 5: dst.mleaf = h.mleaf; break;
+      case 6: dst.ar = rec; dst.cp = h.cp; break;
+      default: dst.nn = nn; dst.lvl = lvl; break;
+    }
   }
 }
 
@@ -158,52 +237,44 @@ C2A_HD void leaf_values(Slot& s, const MeshDev& M1, const MeshDev& M2, const dou
   s.flags = (s.flags & 3) | ((feat[0] != -1 && feat[2] != -1) ? 4 : 0);
 }
 
-// Heap slot h of the depth-K subtree below X: children chosen by the descend
-// rule (C2A.cpp:1195), transforms (C2A.cpp:1199-1226), values
-// (C2A.cpp:1237-1276).
-template <int K, int CAP>
-C2A_HD void compute_slot(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
-                         bool contact) {
-  node_ptr N1 = nodes_of(M1);
-  node_ptr N2 = nodes_of(M2);
-  const int lvl = 31 - clz32((unsigned)(h + 1));
-  double R[9], T[3];
-  for (int i = 0; i < 9; i++) R[i] = g.X.R[i];
-  for (int i = 0; i < 3; i++) T[i] = g.X.T[i];
-  int b1 = g.X.b1, b2 = g.X.b2;
-  Slot& s = g.slot[h];
-  for (int lv = 0; lv < lvl; lv++) {
-    node_ptr n1 = N1 + b1;
-    node_ptr n2 = N2 + b2;
-    const int fc1 = n1->first_child, fc2 = n2->first_child;
-    const bool l1 = fc1 < 0, l2 = fc2 < 0;
-    if (l1 && l2) {
-      s.valid = 0;
-      return;
-    }
-    const int bit = ((h + 1) >> (lvl - 1 - lv)) & 1;
-    double Rn[9], Tn[3], Tt[3], CR[9], CT[3];
-    if (l2 || (!l1 && (n1->size > n2->size))) {
-      const int c = fc1 + bit;
-      node_ptr cn = N1 + c;
-      for (int i = 0; i < 9; i++) CR[i] = cn->R[i];
-      for (int i = 0; i < 3; i++) CT[i] = cn->Tr[i];
-      mtxm(Rn, CR, R);
-      sub3(Tt, T, CT);
-      mtxv(Tn, CR, Tt);
-      b1 = c;
-    } else {
-      const int c = fc2 + bit;
-      node_ptr cn = N2 + c;
-      for (int i = 0; i < 9; i++) CR[i] = cn->R[i];
-      for (int i = 0; i < 3; i++) CT[i] = cn->Tr[i];
-      mxm(Rn, R, CR);
-      mxvpv(Tn, R, CT, T);
-      b2 = c;
-    }
-    for (int i = 0; i < 9; i++) R[i] = Rn[i];
-    for (int i = 0; i < 3; i++) T[i] = Tn[i];
+// One level of descent (C2A.cpp:1195-1226): replaces (b1, b2) and (R, T) by
+// child pair `bit` (0 = the first child node, 1 = its sibling).  Returns
+// false, changing nothing, for a leaf pair.
+C2A_HD bool descend(node_ptr N1, node_ptr N2, int& b1, int& b2, int bit, double* R, double* T) {
+  node_ptr n1 = N1 + b1;
+  node_ptr n2 = N2 + b2;
+  const int fc1 = n1->first_child, fc2 = n2->first_child;
+  const bool l1 = fc1 < 0, l2 = fc2 < 0;
+  if (l1 && l2) return false;
+  double Rn[9], Tn[3], Tt[3], CR[9], CT[3];
+  if (l2 || (!l1 && (n1->size > n2->size))) {
+    const int c = fc1 + bit;
+    node_ptr cn = N1 + c;
+    for (int i = 0; i < 9; i++) CR[i] = cn->R[i];
+    for (int i = 0; i < 3; i++) CT[i] = cn->Tr[i];
+    mtxm(Rn, CR, R);
+    sub3(Tt, T, CT);
+    mtxv(Tn, CR, Tt);
+    b1 = c;
+  } else {
+    const int c = fc2 + bit;
+    node_ptr cn = N2 + c;
+    for (int i = 0; i < 9; i++) CR[i] = cn->R[i];
+    for (int i = 0; i < 3; i++) CT[i] = cn->Tr[i];
+    mxm(Rn, R, CR);
+    mxvpv(Tn, R, CT, T);
+    b2 = c;
   }
+  for (int i = 0; i < 9; i++) R[i] = Rn[i];
+  for (int i = 0; i < 3; i++) T[i] = Tn[i];
+  return true;
+}
+
+// RSS distance and admissible time of pair (b1, b2) at transform (R, T)
+// (C2A.cpp:1237-1276); returns the leaf flags of the two nodes.
+C2A_HD int pair_values(node_ptr N1, node_ptr N2, int b1, int b2, const double* R,
+                       const double* T, const double* r1, const Motion& mo1, const Motion& mo2,
+                       bool contact, double& d_out, double& mt_out) {
   node_ptr A1 = N1 + b1;
   node_ptr A2 = N2 + b2;
   double S[3], la[2] = {A1->l[0], A1->l[1]}, lb[2] = {A2->l[0], A2->l[1]};
@@ -216,21 +287,48 @@ C2A_HD void compute_slot(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDe
       double tmp[3], Sw[3], Sn[3], RL[9];
       for (int i = 0; i < 9; i++) RL[i] = A1->Rloc[i];
       mxv(tmp, RL, S);
-      mxv(Sw, g.r1, tmp);
+      mxv(Sw, r1, tmp);
       neg3(Sn, Sw);
-      double mb1 = motion_bound(g.mo[0], A1->angR, Sw);
-      double mb2 = motion_bound(g.mo[1], A2->angR, Sn);
+      double mb1 = motion_bound(mo1, A1->angR, Sw);
+      double mb2 = motion_bound(mo2, A2->angR, Sn);
       mt = d / (mb1 + mb2);
       if (mt <= 0) mt = 0.0;
     }
   }
+  d_out = d;
+  mt_out = mt;
+  return node_leaf(A1) | (node_leaf(A2) << 1);
+}
+
+// Heap slot h of the depth-K subtree below X: the pair reached by the path
+// bits of h+1, its transform chained from X, and its values.
+template <int K, int CAP>
+C2A_HD void compute_slot(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
+                         bool contact) {
+  node_ptr N1 = nodes_of(M1);
+  node_ptr N2 = nodes_of(M2);
+  const int lvl = 31 - clz32((unsigned)(h + 1));
+  double R[9], T[3];
+  for (int i = 0; i < 9; i++) R[i] = g.X.R[i];
+  for (int i = 0; i < 3; i++) T[i] = g.X.T[i];
+  int b1 = g.X.b1, b2 = g.X.b2;
+  Slot& s = g.slot[h];
+  for (int lv = 0; lv < lvl; lv++) {
+    const int bit = ((h + 1) >> (lvl - 1 - lv)) & 1;
+    if (!descend(N1, N2, b1, b2, bit, R, T)) {
+      s.valid = 0;
+      return;
+    }
+  }
+  double d, mt;
+  const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], contact, d, mt);
   for (int i = 0; i < 9; i++) s.R[i] = R[i];
   for (int i = 0; i < 3; i++) s.T[i] = T[i];
   s.d = d;
   s.mt = mt;
   s.b1 = b1;
   s.b2 = b2;
-  s.flags = node_leaf(A1) | (node_leaf(A2) << 1);
+  s.flags = fl;
   s.valid = 1;
   if ((s.flags & 3) == 3)
     leaf_values(s, M1, M2, g.Rrel, g.Trel, g.r1, g.tt1, g.mo[0], g.mo[1], b1, b2, contact);
@@ -258,7 +356,7 @@ C2A_HD void round_lane(Wave<K, CAP>& g, int lane, int nl, const KArgs& a) {
 }
 
 template <int K, int CAP>
-C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, int lane, int nl) {
+C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
   Entry& e = g.stack[L.sp - 1];
   int gen = g.gen + 1;
   if (gen >= 0x7fffff) {   // generation wrap: forget every cached link
@@ -266,11 +364,19 @@ C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, int lane, int nl) {
     gen = 1;
   }
   const int flags = e.flags;
+  const int ar = e.ar;
   const int sg = (gen << 8) | 1;
   lane_sync();
   const uint64_t* s = reinterpret_cast<const uint64_t*>(&e);
   uint64_t* x = reinterpret_cast<uint64_t*>(&g.X);
-  for (int w = lane; w < 16; w += nl) x[w] = s[w];
+  if (flags & 8) {
+    // replayed pair: the transform is in the arena
+    const uint64_t C2A_GLOBAL* xf = gp(reinterpret_cast<const uint64_t*>(
+        arena_view(L.ab, a.logcap).xf + 12 * (int64_t)ar));
+    for (int w = lane; w < 16; w += nl) x[w] = w < 12 ? xf[w] : s[w];
+  } else {
+    for (int w = lane; w < 16; w += nl) x[w] = s[w];
+  }
   lane_sync();
   if (lane == 0) {
     e.sg = sg;
@@ -287,11 +393,13 @@ C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, int lane, int nl) {
   L.need_round = 1;
 }
 
-// Root pair of a traversal at absolute poses (C2A.cpp:1793-1806).
+// Root pair of a traversal at absolute poses (C2A.cpp:1793-1806).  `nn` is
+// the root's record number in this step's log (0), or -1 when the traversal
+// is not logged.
 template <int K, int CAP>
 C2A_HD void start_traversal(Wave<K, CAP>& g, Leader& L, const MeshDev& M1, const MeshDev& M2,
                             const double* R1, const double* T1, const double* R2,
-                            const double* T2, int lane) {
+                            const double* T2, int nn, int lane) {
   double Tt[3], Rt[9], R[9], T[3], Rrel[9], Trel[3], C1R[9], C2R[9], C1T[3], C2T[3];
   node_ptr r1n = nodes_of(M1);
   node_ptr r2n = nodes_of(M2);
@@ -319,18 +427,139 @@ C2A_HD void start_traversal(Wave<K, CAP>& g, Leader& L, const MeshDev& M1, const
     e.b2 = 0;
     e.sg = 0;
     e.flags = node_leaf(r1n) | (node_leaf(r2n) << 1) | 4;
+    e.dTri = 0.0;
+    e.mleaf = 0.0;
+    e.ar = -1;
+    e.cp = -1;
+    e.nn = nn;
+    e.lvl = 0;
   }
   lane_sync();
   L.sp = 1;
+}
+
+// Recomputes, at this step's poses, every pair the previous step's walk
+// computed as a child of an expansion (C2A.cpp:1195-1276 per pair), level by
+// level so that each record's parent transform is ready before it is needed.
+// Lanes take one record each.
+template <int K, int CAP>
+C2A_HD void replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
+  const ArenaView av = arena_view(L.ab, a.logcap);
+  const LogEnt C2A_GLOBAL* lg = gp(L.logbuf ? av.log0 : av.log1);
+  double C2A_GLOBAL* xf = gp(av.xf);
+  int32_t C2A_GLOBAL* cpa = gp(av.cpa);
+  int32_t C2A_GLOBAL* order = gp(av.order);
+  HotRec C2A_GLOBAL* hot = gp(av.hot);
+  const int n = L.nlog_prev;
+  const int nrec = 1 + 2 * n;
+  const MeshDev M1 = a.meshes[g.mesh[0]];
+  const MeshDev M2 = a.meshes[g.mesh[1]];
+  node_ptr N1 = nodes_of(M1);
+  node_ptr N2 = nodes_of(M2);
+  mem_sync();   // the previous walk's log stores
+  // level histogram, children links
+  for (int i = lane; i < CAP + 2; i += nl) g.lh[i] = 0;
+  for (int i = lane; i < nrec; i += nl) cpa[i] = -1;
+  mem_sync();
+  for (int e = lane; e < n; e += nl) lds_add(&g.lh[lg[e].level], 1);
+  for (int e = lane; e < n; e += nl) cpa[lg[e].rec] = e;
+  lane_sync();
+  if (lane == 0) {
+    int acc = 0, top = 0;
+    for (int v = 0; v < CAP + 2; v++) {
+      g.ls[v] = acc;
+      const int c = g.lh[v];
+      if (c) top = v + 1;
+      g.lh[v] = acc;
+      acc += c;
+    }
+    g.ls[CAP + 2] = top;   // number of levels in use
+    // root record: the transform start_traversal placed in the root entry
+    for (int i = 0; i < 9; i++) xf[i] = g.stack[0].R[i];
+    for (int i = 0; i < 3; i++) xf[9 + i] = g.stack[0].T[i];
+  }
+  lane_sync();
+  for (int e = lane; e < n; e += nl) {
+    const int pos = lds_add(&g.lh[lg[e].level], 1);
+    order[pos] = e;
+  }
+  mem_sync();
+  const int top = g.ls[CAP + 2];
+  for (int v = 0; v < top; v++) {
+    const int s0 = g.ls[v];
+    const int cnt = (v + 1 < CAP + 2 ? g.ls[v + 1] : n) - s0;
+    for (int j = lane; j < 2 * cnt; j += nl) {
+      const int e = order[s0 + (j >> 1)];
+      const int side = j & 1;
+      const LogEnt le = lg[e];
+      const int rec = 2 * e + 1 + side;
+      double R[9], T[3];
+      const double C2A_GLOBAL* pxf = xf + 12 * (int64_t)le.rec;
+      for (int i = 0; i < 9; i++) R[i] = pxf[i];
+      for (int i = 0; i < 3; i++) T[i] = pxf[9 + i];
+      int b1 = le.b1, b2 = le.b2;
+      descend(N1, N2, b1, b2, side, R, T);   // an expanded pair is never a leaf pair
+      double d, mt;
+      const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], false, d, mt);
+      double C2A_GLOBAL* oxf = xf + 12 * (int64_t)rec;
+      for (int i = 0; i < 9; i++) oxf[i] = R[i];
+      for (int i = 0; i < 3; i++) oxf[9 + i] = T[i];
+      HotRec h;
+      h.d = d;
+      h.mt = mt;
+      h.b1 = b1;
+      h.b2 = b2;
+      h.flags = fl;
+      h.cp = cpa[rec];
+      h.dTri = 0.0;
+      h.mleaf = 0.0;
+      h.t1 = -1;
+      h.t2 = -1;
+      h.pad[0] = h.pad[1] = 0;
+      if (fl == 3) {
+        Slot tmp;
+        tmp.flags = 3;
+        leaf_values(tmp, M1, M2, g.Rrel, g.Trel, g.r1, g.tt1, g.mo[0], g.mo[1], b1, b2, false);
+        h.dTri = tmp.dTri;
+        h.mleaf = tmp.mleaf;
+        h.t1 = tmp.t1;
+        h.t2 = tmp.t2;
+      }
+      hot[rec] = h;
+    }
+    mem_sync();
+  }
+  if (lane == 0) {
+    Entry& r = g.stack[0];
+    r.flags |= 8;
+    r.ar = 0;
+    r.cp = cpa[0];
+    g.wbase = -(1 << 30);
+  }
+  lane_sync();
+}
+
+// Stages replayed records [rec, rec + WIN) in LDS.
+template <int K, int CAP>
+C2A_HD void load_window(Wave<K, CAP>& g, const ArenaView& av, int rec, int nrec_max, int lane,
+                        int nl) {
+  lane_sync();
+  const HotRec C2A_GLOBAL* hot = gp(av.hot);
+  for (int i = lane; i < WIN; i += nl)
+    if (rec + i < nrec_max) g.win[i] = hot[rec + i];
+  if (lane == 0) g.wbase = rec;
+  lane_sync();
 }
 
 // The reference DFS as a stack machine (C2A.cpp:1113-1354 and, in contact
 // mode, 1525-1725).  Uniform across lanes.  Returns true when a round has
 // been requested.
 template <int K, int CAP>
-C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, int lane, int nl) {
+C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
   const bool contact = g.contact != 0;
   const int gen = g.gen;
+  const ArenaView av = arena_view(L.ab, a.logcap);
+  const int nrec_max = 1 + 2 * a.logcap;
   while (L.sp > 0) {
     const Entry& e = g.stack[L.sp - 1];
     const int flags = e.flags;
@@ -349,8 +578,24 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, int lane, int nl) {
     const bool cached = (sg >> 8) == gen && (sg & 0xff) != 0;
     const int s = (sg & 0xff) - 1;
     if ((flags & 3) == 3) {
+      if (flags & 8) {
+        // replayed leaf pair (C2A.cpp:1141-1183); p1/p2 are recomputed at the
+        // end of the step if this stays the last improving leaf
+        const double dTri = e.dTri;
+        if (dTri <= L.D) {
+          L.D = dTri;
+          L.pq_pending = 1;
+          if (e.mleaf <= L.M) L.M = e.mleaf;
+          L.last1 = e.b1;
+          L.last2 = e.b2;
+        }
+        L.tri++;
+        L.sp--;
+        continue;
+      }
       if (!cached) {
-        request_round(g, L, lane, nl);
+        C2A_EMU_COUNT(0, L);
+        request_round(g, L, a, lane, nl);
         return true;
       }
       const Slot& sl = g.slot[s];
@@ -358,6 +603,7 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, int lane, int nl) {
         const double dTri = sl.dTri;
         if (dTri <= L.D) {
           L.D = dTri;
+          L.pq_pending = 0;
           for (int i = 0; i < 3; i++) { L.p1[i] = sl.p[i]; L.p2[i] = sl.q[i]; }
           if (sl.mleaf <= L.M) L.M = sl.mleaf;
           L.last1 = sl.t1;
@@ -370,37 +616,97 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, int lane, int nl) {
       L.sp--;
       continue;
     }
+    const int cp = e.cp;
+    const bool from_arena = (flags & 8) && cp >= 0;
     const int lvl = cached ? 31 - clz32((unsigned)(s + 1)) : K;
-    if (!cached || lvl >= K) {
-      request_round(g, L, lane, nl);
+    if (!from_arena && (!cached || lvl >= K)) {
+      C2A_EMU_COUNT((flags & 8) ? 1 : (cached ? 2 : 3), L);
+      request_round(g, L, a, lane, nl);
       return true;
     }
-    const int sa = 2 * s + 1, sc = 2 * s + 2;
     if (!contact) L.bv += 2;
+    // log the expansion for the next step's replay
+    int nnA = -1;
+    const int enn = e.nn, elvl = e.lvl;
+    if (enn >= 0 && L.nlog < a.logcap) {
+      const int en = L.nlog++;
+      if (lane == 0) {
+        LogEnt C2A_GLOBAL* le = gp(L.logbuf ? av.log1 : av.log0) + en;
+        LogEnt v;
+        v.rec = enn;
+        v.b1 = e.b1;
+        v.b2 = e.b2;
+        v.level = elvl;
+        *le = v;
+      }
+      nnA = 2 * en + 1;
+    }
+    const int nnC = nnA < 0 ? -1 : nnA + 1;
     L.sp--;
     if (L.sp + 2 > CAP) {
       L.flags |= C2A_FLAG_STACK_OVERFLOW;
       L.sp = 0;
       return false;
     }
+    if (from_arena) {
+      const int ra = 2 * cp + 1;
+      if (ra < g.wbase || ra + 1 >= g.wbase + WIN) load_window(g, av, ra, nrec_max, lane, nl);
+      const HotRec& A = g.win[ra - g.wbase];
+      const HotRec& C = g.win[ra + 1 - g.wbase];
+      // closer child visited first (C2A.cpp:1281): it goes on top
+      const bool c_first = C.d < A.d;
+      lane_sync();
+      if (c_first) {
+        hot_to_entry(g.stack[L.sp], A, ra, nnA, elvl + 1, lane, nl);
+        hot_to_entry(g.stack[L.sp + 1], C, ra + 1, nnC, elvl + 1, lane, nl);
+      } else {
+        hot_to_entry(g.stack[L.sp], C, ra + 1, nnC, elvl + 1, lane, nl);
+        hot_to_entry(g.stack[L.sp + 1], A, ra, nnA, elvl + 1, lane, nl);
+      }
+      lane_sync();
+      L.sp += 2;
+      continue;
+    }
+    const int sa = 2 * s + 1, sc = 2 * s + 2;
     const Slot& A = g.slot[sa];
     const Slot& C = g.slot[sc];
     const int sgA = (gen << 8) | (sa + 1), sgC = (gen << 8) | (sc + 1);
     const int fA = A.flags & 3, fC = C.flags & 3;
-    // closer child visited first (C2A.cpp:1281): it goes on top
     const bool c_first = C.d < A.d;
     lane_sync();
     if (c_first) {
-      slot_to_entry(g.stack[L.sp], A, sgA, fA, lane, nl);
-      slot_to_entry(g.stack[L.sp + 1], C, sgC, fC, lane, nl);
+      slot_to_entry(g.stack[L.sp], A, sgA, fA, nnA, elvl + 1, lane, nl);
+      slot_to_entry(g.stack[L.sp + 1], C, sgC, fC, nnC, elvl + 1, lane, nl);
     } else {
-      slot_to_entry(g.stack[L.sp], C, sgC, fC, lane, nl);
-      slot_to_entry(g.stack[L.sp + 1], A, sgA, fA, lane, nl);
+      slot_to_entry(g.stack[L.sp], C, sgC, fC, nnC, elvl + 1, lane, nl);
+      slot_to_entry(g.stack[L.sp + 1], A, sgA, fA, nnA, elvl + 1, lane, nl);
     }
     lane_sync();
     L.sp += 2;
   }
   return false;
+}
+
+// After a step's walk: closest points of a replayed last improving leaf
+// (the same triangle-distance call a round would have made), and the log
+// hand-over to the next step.
+template <int K, int CAP>
+C2A_HD void end_walk(Wave<K, CAP>& g, Leader& L, const KArgs& a) {
+  if (L.pq_pending) {
+    const MeshDev M1 = a.meshes[g.mesh[0]];
+    const MeshDev M2 = a.meshes[g.mesh[1]];
+    double ta[9], tb[9], Rrel[9], Trel[3];
+    gdouble_ptr g1 = tris_of(M1) + 9 * (int64_t)L.last1;
+    gdouble_ptr g2 = tris_of(M2) + 9 * (int64_t)L.last2;
+    for (int i = 0; i < 9; i++) { ta[i] = g1[i]; tb[i] = g2[i]; Rrel[i] = g.Rrel[i]; }
+    for (int i = 0; i < 3; i++) Trel[i] = g.Trel[i];
+    (void)tri_distance(Rrel, Trel, ta, tb, L.p1, L.p2, nullptr);
+    L.pq_pending = 0;
+  }
+  if (a.logcap > 0) {
+    L.nlog_prev = L.nlog;
+    L.logbuf ^= 1;
+  }
 }
 
 template <int K, int CAP>
@@ -415,7 +721,9 @@ C2A_HD void begin_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     motion_integrate(g.mo[0], L.lamda, R1, T1, nullptr);
     motion_integrate(g.mo[1], L.lamda, R2, T2, nullptr);
   }
-  start_traversal(g, L, M1, M2, R1, T1, R2, T2, lane);
+  start_traversal(g, L, M1, M2, R1, T1, R2, T2, a.logcap > 0 ? 0 : -1, lane);
+  L.nlog = 0;
+  L.pq_pending = 0;
   // initial bound from the caller-seeded pair (C2A.cpp:1812-1858)
   double p[3], q[3], ta[9], tb[9];
   gdouble_ptr g1 = tris_of(M1) + 9 * (int64_t)L.seed1;
@@ -471,7 +779,7 @@ C2A_HD void finish_query(Wave<K, CAP>& g, Leader& L, const KArgs& a, bool collis
     motion_integrate(g.mo[1], L.toc, R2, T2, nullptr);
     L.thr = a.contact_scale * L.D + a.contact_offset;
     L.num_contact = 0;
-    start_traversal(g, L, M1, M2, R1, T1, R2, T2, lane);
+    start_traversal(g, L, M1, M2, R1, T1, R2, T2, -1, lane);
     if (lane == 0) g.contact = 1;
     lane_sync();
     L.state = ST_CONTACT_WALK;
@@ -592,7 +900,9 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     double R1[9], T1[3], R2[9], T2[3];
     for (int i = 0; i < 9; i++) { R1[i] = q.R1[i]; R2[i] = q.R2[i]; }
     for (int i = 0; i < 3; i++) { T1[i] = q.T1[i]; T2[i] = q.T2[i]; }
-    start_traversal(g, L, M1, M2, R1, T1, R2, T2, lane);
+    start_traversal(g, L, M1, M2, R1, T1, R2, T2, -1, lane);
+    L.nlog = 0;
+    L.pq_pending = 0;
     double p[3], qq[3], ta[9], tb[9], Rrel[9], Trel[3];
     gdouble_ptr g1 = tris_of(M1) + 9 * (int64_t)L.seed1;
     gdouble_ptr g2 = tris_of(M2) + 9 * (int64_t)L.seed2;
@@ -627,7 +937,7 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     double R1[9], T1[3], R2[9], T2[3];
     for (int i = 0; i < 9; i++) { R1[i] = c.R1[i]; R2[i] = c.R2[i]; }
     for (int i = 0; i < 3; i++) { T1[i] = c.T1[i]; T2[i] = c.T2[i]; }
-    start_traversal(g, L, M1, M2, R1, T1, R2, T2, lane);
+    start_traversal(g, L, M1, M2, R1, T1, R2, T2, -1, lane);
     L.state = ST_CONTACT_WALK;
     return true;
   }
@@ -644,6 +954,8 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   L.last1 = q.seed[0];   // model last_tri starts at the seed (demo protocol)
   L.last2 = q.seed[1];
   L.mode = MODE_QUERY;
+  L.nlog_prev = 0;
+  L.logbuf = 0;
   L.numCA = 0;
   L.lamda = 0.0;
   L.lastLamda = 0.0;
@@ -666,14 +978,16 @@ C2A_HD void leader_run(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int
         break;
       case ST_STEP:
         begin_step(g, L, a, lane);
+        if (a.logcap > 0 && L.nlog_prev > 0) replay(g, L, a, lane, nl);
         L.state = ST_WALK;
         break;
       case ST_WALK:
-        if (walk(g, L, lane, nl)) return;
+        if (walk(g, L, a, lane, nl)) return;
+        end_walk(g, L, a);
         after_step(g, L, a, lane);
         break;
       case ST_CONTACT_WALK:
-        if (walk(g, L, lane, nl)) return;
+        if (walk(g, L, a, lane, nl)) return;
         if (lane == 0) {
           if (L.mode == MODE_CONTACT) a.counts_out[L.qi] = L.num_contact;
           else a.out[L.qi].num_contact = L.num_contact;

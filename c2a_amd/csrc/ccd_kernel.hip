@@ -24,6 +24,7 @@ __global__ void __launch_bounds__(64) ccd_kernel(KArgs a) {
   Leader L;
   L.state = ST_FETCH;
   L.sp = 0;
+  L.ab = a.logcap > 0 ? a.arena + (size_t)blockIdx.x * arena_bytes(a.logcap) : nullptr;
   for (;;) {
     leader_run(g, L, a, lane, 64);
     if (L.state == ST_DONE) break;

@@ -46,6 +46,63 @@ struct ContactDev {
   int32_t mesh[2];
 };
 
+// Replay of the previous CA step.  Consecutive steps of one query visit
+// nearly the same pairs, so the walk logs every expansion and the next step
+// recomputes the logged subtree in parallel, level by level, before it walks.
+// Record numbering of a step: the root pair is record 0 and the two children
+// of logged expansion e are records 2e+1 (first child node) and 2e+2 (its
+// sibling).
+struct LogEnt {          // one expansion of a walk
+  int32_t rec;           // record number of the expanded pair
+  int32_t b1, b2;        // its node ids
+  int32_t level;         // its BVTT depth (root 0)
+};
+
+struct HotRec {          // what the walk reads of a replayed record (64 B)
+  double d, mt;          // RSS distance and admissible time
+  double dTri, mleaf;    // leaf pairs: triangle distance and leaf admissible time
+  int32_t b1, b2;        // node ids
+  int32_t flags;         // bit0 leaf node of model 1, bit1 leaf node of model 2
+  int32_t cp;            // expansion whose children are this pair's children, -1 none
+  int32_t t1, t2;        // leaf pairs: triangle indices
+  int32_t pad[2];
+};
+
+// Per-wavefront replay storage, carved from one device allocation.
+struct ArenaView {
+  LogEnt* log0;          // written by the walk / read by the next replay, alternating
+  LogEnt* log1;
+  double* xf;            // nrec x 12: pair transform R[9], T[3]
+  HotRec* hot;           // nrec
+  int32_t* cpa;          // nrec: expansion index of each record's children
+  int32_t* order;        // logcap: expansions sorted by level
+};
+
+C2A_HD size_t arena_bytes(int logcap) {
+  const size_t nrec = 1 + 2 * (size_t)logcap;
+  size_t b = 2 * (size_t)logcap * sizeof(LogEnt) + nrec * 12 * sizeof(double) +
+             nrec * sizeof(HotRec) + nrec * sizeof(int32_t) + (size_t)logcap * sizeof(int32_t);
+  return (b + 255) & ~(size_t)255;
+}
+
+C2A_HD ArenaView arena_view(unsigned char* base, int logcap) {
+  const size_t nrec = 1 + 2 * (size_t)logcap;
+  ArenaView v;
+  unsigned char* p = base;
+  v.log0 = (LogEnt*)p;
+  p += (size_t)logcap * sizeof(LogEnt);
+  v.log1 = (LogEnt*)p;
+  p += (size_t)logcap * sizeof(LogEnt);
+  v.xf = (double*)p;
+  p += nrec * 12 * sizeof(double);
+  v.hot = (HotRec*)p;
+  p += nrec * sizeof(HotRec);
+  v.cpa = (int32_t*)p;
+  p += nrec * sizeof(int32_t);
+  v.order = (int32_t*)p;
+  return v;
+}
+
 struct KArgs {
   const MeshDev* meshes;
   const QueryDev* q;
@@ -61,7 +118,8 @@ struct KArgs {
   int32_t max_iter;
   int32_t with_contacts;
   int32_t mode;
-  int32_t pad;
+  int32_t logcap;          // expansions logged per step per wavefront, 0 = no replay
+  unsigned char* arena;    // one arena_bytes(logcap) slice per workgroup
 };
 
 }  // namespace c2a

@@ -64,6 +64,9 @@ typedef struct {
   double contact_offset;
   int32_t spec_depth;       /* speculative subtree depth per round, 0 = auto */
   int32_t groups_per_wave;  /* queries per wavefront, 0 = auto */
+  int32_t replay_log;       /* expansions logged per CA step for the next step's
+                               replay (default 8192); 0 disables replay */
+  int32_t pad;
 } c2a_params;
 
 /* Per-query result (C2A_TimeOfContactResult, C2A.h:161-224, plus the

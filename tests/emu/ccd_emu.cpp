@@ -7,13 +7,21 @@
 #include <memory>
 #include <vector>
 
+#include <stdint.h>
+static int64_t g_kind[2][4];   // rounds by reason, first step / later steps
+#define C2A_EMU_COUNT(kind, L) (g_kind[(L).numCA > 1 ? 1 : 0][kind]++)
 #include "../../c2a_amd/csrc/bvh.h"
 #include "../../c2a_amd/csrc/ccd_core.h"
 
 using namespace c2a;
 
+static int64_t g_rounds = 0;   // speculative rounds of the last batch
+extern "C" int64_t emu_rounds() { return g_rounds; }
+extern "C" int64_t emu_round_kind(int later, int kind) { return g_kind[later][kind]; }
+
 template <int K>
 static int run(const KArgs& a) {
+  std::vector<unsigned char> arena(a.logcap > 0 ? arena_bytes(a.logcap) : 1);
   using W = Wave<K, 96>;
   std::unique_ptr<W> gp(new W());
   W& g = *gp;
@@ -21,9 +29,13 @@ static int run(const KArgs& a) {
   Leader L;
   memset(&L, 0, sizeof(L));
   L.state = ST_FETCH;
+  L.ab = a.logcap > 0 ? arena.data() : nullptr;
+  g_rounds = 0;
+  memset(g_kind, 0, sizeof(g_kind));
   for (;;) {
     leader_run(g, L, a, 0, 1);
     if (L.state == ST_DONE) break;
+    g_rounds++;
     round_lane(g, 0, 1, a);
   }
   return 0;
@@ -66,6 +78,7 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
   a.max_iter = p->max_iter;
   a.with_contacts = p->with_contacts;
   a.mode = MODE_QUERY;
+  a.logcap = p->replay_log;
   switch (K) {
     case 1: return run<1>(a);
     case 2: return run<2>(a);
