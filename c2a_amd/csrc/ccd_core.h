@@ -72,6 +72,27 @@ C2A_HD int lds_add(int32_t* p, int v) {
 #endif
 }
 
+// Values every lane holds identically (read from LDS at one address): moves
+// them to scalar registers so the walk branches on scalars.
+C2A_HD int32_t uni(int32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_readfirstlane(x);
+#else
+  return x;
+#endif
+}
+C2A_HD double unid(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  union { double d; int32_t i[2]; } u;
+  u.d = x;
+  u.i[0] = __builtin_amdgcn_readfirstlane(u.i[0]);
+  u.i[1] = __builtin_amdgcn_readfirstlane(u.i[1]);
+  return u.d;
+#else
+  return x;
+#endif
+}
+
 // Node data is read-only global memory; the address-space qualifier lets the
 // compiler emit global (not flat) loads.
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -103,7 +124,61 @@ struct Entry {           // 160 B stack entry: a considered-but-undecided pair
   int32_t nn, lvl;       // record number in this step's log (-1 = unlogged), depth
 };
 
+// Words 12-19 of an Entry: everything the walk decides on, held in registers
+// for the pair being processed.
+struct Cur {
+  double d, mt;
+  int32_t b1, b2;
+  int32_t sg, flags;
+  double dTri, mleaf;
+  int32_t ar, cp;
+  int32_t nn, lvl;
+};
+
+C2A_HD Cur read_cur(const Entry& e) {
+  Cur c = *reinterpret_cast<const Cur*>(&e.d);
+  c.d = unid(c.d);
+  c.mt = unid(c.mt);
+  c.b1 = uni(c.b1);
+  c.b2 = uni(c.b2);
+  c.sg = uni(c.sg);
+  c.flags = uni(c.flags);
+  c.dTri = unid(c.dTri);
+  c.mleaf = unid(c.mleaf);
+  c.ar = uni(c.ar);
+  c.cp = uni(c.cp);
+  c.nn = uni(c.nn);
+  c.lvl = uni(c.lvl);
+  return c;
+}
+
+C2A_HD void write_cur(Entry& e, const Cur& c, int lane) {
+  if (lane == 0) *reinterpret_cast<Cur*>(&e.d) = c;
+}
+
+// A replayed record as the walk holds it: leaf pairs carry their triangle
+// indices in b1/b2.
+C2A_HD Cur cur_from_hot(const HotRec& h0, int rec, int nn, int lvl) {
+  Cur c;
+  const int fl = uni(h0.flags);
+  const bool leaf = (fl & 3) == 3;
+  c.d = unid(h0.d);
+  c.mt = unid(h0.mt);
+  c.b1 = uni(leaf ? h0.t1 : h0.b1);
+  c.b2 = uni(leaf ? h0.t2 : h0.b2);
+  c.sg = 0;
+  c.flags = (fl & 3) | 8;
+  c.dTri = unid(h0.dTri);
+  c.mleaf = unid(h0.mleaf);
+  c.ar = rec;
+  c.cp = uni(h0.cp);
+  c.nn = nn;
+  c.lvl = lvl;
+  return c;
+}
+
 constexpr int WIN = 64;  // replayed records staged in LDS for the walk
+constexpr int LBUF = 64; // log entries staged in LDS before they go to the arena
 
 struct Slot {            // speculative pair values, heap-indexed below X
   double R[9], T[3];     // same first 14 words as Entry
@@ -133,13 +208,19 @@ struct Wave {
   int32_t mesh[2];
   int32_t contact;           // 1: contact-enumeration traversal (no motion bounds)
   int32_t gen;
-  HotRec win[WIN];           // replayed records [wbase, wbase + WIN)
-  int32_t wbase;
+  HotRec win[WIN];           // replayed records [L.wbase, L.wbase + WIN)
+  LogEnt lbuf[LBUF];         // log entries [L.nflush, L.nlog) not yet in the arena
+  int32_t anc[CAP + 2];      // scan: log index of the last event at each depth
+  int32_t lastlane[CAP + 2]; // scan: scratch for the per-depth last event
   int32_t lh[CAP + 2];       // replay: expansions per level, then fill cursors
   int32_t ls[CAP + 3];       // replay: first sorted expansion of each level
 };
 
-enum : int { ST_FETCH = 0, ST_STEP = 1, ST_WALK = 2, ST_CONTACT_WALK = 3, ST_DONE = 4 };
+enum : int {
+  ST_FETCH = 0, ST_STEP = 1, ST_WALK = 2, ST_CONTACT_WALK = 3, ST_DONE = 4,
+  ST_SCAN = 5,   // lane-parallel walk over the replayed subtree
+  ST_SUB = 6     // sequential walk below a pair the scan found unexpanded
+};
 
 // The CA loop and step state of one query; every lane holds the same values.
 struct Leader {
@@ -160,7 +241,10 @@ struct Leader {
   double thr;
   int mode;
   int nlog, nlog_prev, logbuf;   // expansions logged this step / by the previous step
+  int nflush;                    // log entries already copied to the arena
   int pq_pending;                // p1, p2 of the last improving leaf still to compute
+  int wbase;                     // first record staged in the LDS window
+  int spos, nev, logfull;        // scan position, events, log capacity reached
   unsigned char* ab;             // this wavefront's replay arena
 };
 
@@ -179,26 +263,6 @@ C2A_HD void slot_to_entry(Entry& dst, const Slot& src, int sg, int flags, int nn
     else if (w == 17) dst.mleaf = 0.0;
     else if (w == 18) { dst.ar = -1; dst.cp = -1; }
     else { dst.nn = nn; dst.lvl = lvl; }
-  }
-}
-
-// Stack entry for replayed record `rec` (values from the LDS window).  The
-// transform stays in the arena; leaf pairs carry their triangle indices in
-// b1/b2.
-C2A_HD void hot_to_entry(Entry& dst, const HotRec& h, int rec, int nn, int lvl, int lane,
-                         int nl) {
-  const bool leaf = (h.flags & 3) == 3;
-  for (int w = lane; w < 8; w += nl) {
-    switch (w) {
-      case 0: dst.d = h.d; break;
-      case 1: dst.mt = h.mt; break;
-      case 2: dst.b1 = leaf ? h.t1 : h.b1; dst.b2 = leaf ? h.t2 : h.b2; break;
-      case 3: dst.sg = 0; dst.flags = (h.flags & 3) | 8; break;
-      case 4: dst.dTri = h.dTri; break;
-      case 5: dst.mleaf = h.mleaf; break;
-      case 6: dst.ar = rec; dst.cp = h.cp; break;
-      default: dst.nn = nn; dst.lvl = lvl; break;
-    }
   }
 }
 
@@ -529,169 +593,483 @@ C2A_HD void replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl)
     }
     mem_sync();
   }
+  // Scan layout: replayed subtree sizes bottom-up, then depth-first
+  // positions top-down in this step's visiting order (closer child first,
+  // C2A.cpp:1281), then the records scattered into that order.
+  int32_t C2A_GLOBAL* sz = gp(av.sz);
+  int32_t C2A_GLOBAL* pos = gp(av.pos);
+  Event C2A_GLOBAL* ev = gp(av.ev);
+  for (int i = lane; i < nrec; i += nl) sz[i] = 1;
+  mem_sync();
+  for (int v = top - 1; v >= 0; v--) {
+    const int s0 = g.ls[v];
+    const int cnt = (v + 1 < CAP + 2 ? g.ls[v + 1] : n) - s0;
+    for (int j = lane; j < cnt; j += nl) {
+      const int e = order[s0 + j];
+      sz[lg[e].rec] = 1 + sz[2 * e + 1] + sz[2 * e + 2];
+    }
+    mem_sync();
+  }
+  if (lane == 0) pos[0] = -1;
+  mem_sync();
+  for (int v = 0; v < top; v++) {
+    const int s0 = g.ls[v];
+    const int cnt = (v + 1 < CAP + 2 ? g.ls[v + 1] : n) - s0;
+    for (int j = lane; j < cnt; j += nl) {
+      const int e = order[s0 + j];
+      const int p0 = pos[lg[e].rec] + 1;
+      const int rA = 2 * e + 1, rC = 2 * e + 2;
+      const bool c_first = hot[rC].d < hot[rA].d;
+      const int rf = c_first ? rC : rA, rs = c_first ? rA : rC;
+      pos[rf] = p0;
+      pos[rs] = p0 + sz[rf];
+    }
+    mem_sync();
+  }
+  for (int r = lane + 1; r < nrec; r += nl) {
+    const LogEnt le = lg[(r - 1) >> 1];
+    const HotRec h = hot[r];
+    const bool leaf = (h.flags & 3) == 3;
+    Event E;
+    E.d = h.d;
+    E.mt = h.mt;
+    E.dTri = h.dTri;
+    E.mleaf = h.mleaf;
+    E.b1 = leaf ? h.t1 : h.b1;
+    E.b2 = leaf ? h.t2 : h.b2;
+    E.rec = r;
+    E.size = sz[r];
+    E.ppos = pos[le.rec];
+    E.info = (h.flags & 3) | (h.cp >= 0 ? 4 : 0) | (((r - 1) & 1) << 3) | ((le.level + 1) << 8);
+    ev[pos[r]] = E;
+  }
+  mem_sync();
+  L.wbase = -(1 << 30);
+  lane_sync();
+}
+
+// Wavefront primitives of the scan (the host harness runs one lane).
+C2A_HD uint64_t wave_ballot(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (uint64_t)__ballot(p);
+#else
+  return p ? 1u : 0u;
+#endif
+}
+C2A_HD int lanes_below(uint64_t m, int lane) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  (void)lane;
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+#else
+  return __builtin_popcountll(m & ((1ull << lane) - 1));
+#endif
+}
+C2A_HD int lane_bcast(int v, int src) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __shfl(v, src);
+#else
+  (void)src;
+  return v;
+#endif
+}
+C2A_HD double lane_bcastd(double v, int src) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __shfl(v, src);
+#else
+  (void)src;
+  return v;
+#endif
+}
+// max over lanes below this one (0 for lane 0)
+C2A_HD int excl_prefix_max(int v, int lane) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, (unsigned)o);
+    if (lane >= o) v = v > u ? v : u;
+  }
+  const int x = __shfl_up(v, 1u);
+  return lane == 0 ? 0 : x;
+#else
+  (void)v;
+  (void)lane;
+  return 0;
+#endif
+}
+C2A_HD double wave_mind(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double u = __shfl_xor(v, o);
+    v = u < v ? u : v;
+  }
+#endif
+  return v;
+}
+C2A_HD int wave_maxi(int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int o = 32; o >= 1; o >>= 1) {
+    const int u = __shfl_xor(v, o);
+    v = u > v ? u : v;
+  }
+#endif
+  return v;
+}
+C2A_HD int lds_max(int32_t* p, int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicMax(p, v);
+#else
+  int o = *p;
+  if (v > o) *p = v;
+  return o;
+#endif
+}
+
+// Starts the scan of a replayed step: the root pair is always expanded
+// (C2A.cpp:1920 -> 1279) and logged first.
+template <int K, int CAP>
+C2A_HD void start_scan(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
+  const ArenaView av = arena_view(L.ab, a.logcap);
   if (lane == 0) {
-    Entry& r = g.stack[0];
-    r.flags |= 8;
-    r.ar = 0;
-    r.cp = cpa[0];
-    g.wbase = -(1 << 30);
+    LogEnt v;
+    v.rec = 0;
+    v.b1 = 0;
+    v.b2 = 0;
+    v.level = 0;
+    gp(L.logbuf ? av.log1 : av.log0)[0] = v;
+  }
+  for (int i = lane; i < CAP + 2; i += nl) {
+    g.anc[i] = i == 0 ? 0 : -1;
+    g.lastlane[i] = -1;
   }
   lane_sync();
+  L.nlog = 1;
+  L.nflush = 1;
+  L.logfull = 0;
+  L.bv += 2;
+  L.spos = 0;
+  L.nev = 2 * L.nlog_prev;
+}
+
+// The reference DFS over the replayed subtree, 64 decisions at a time.  The
+// events are the replayed pairs in this step's depth-first order; each lane
+// decides one with the current bound (C2A.cpp:1284-1349).  The running
+// distance only changes at an improving leaf, so every decision before the
+// first improving leaf is final; a pair the previous step did not expand is
+// handed to the sequential walk.  A pruned pair the previous step expanded
+// skips its replayed subtree.  Returns 1 when such a pair waits on the
+// stack, 0 when the step is complete.
+template <int K, int CAP>
+C2A_HD int scan(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
+  const ArenaView av = arena_view(L.ab, a.logcap);
+  const Event C2A_GLOBAL* ev = gp(av.ev);
+  LogEnt C2A_GLOBAL* lw = gp(L.logbuf ? av.log1 : av.log0);
+  const double inf = 1.0 / 0.0;
+  while (L.spos < L.nev) {
+    const int t0 = L.spos;
+    const int n = (L.nev - t0) < nl ? (L.nev - t0) : nl;
+    const int j = lane;
+    const bool valid = j < n;
+    Event E;
+    if (valid) {
+      E = ev[t0 + j];
+    } else {
+      E.d = E.mt = E.dTri = E.mleaf = 0.0;
+      E.b1 = E.b2 = E.rec = E.ppos = 0;
+      E.size = 1;
+      E.info = 3;
+    }
+    const int lev = E.info >> 8;
+    const bool internal = (E.info & 3) != 3;
+    const bool hasch = (E.info & 4) != 0;
+    const bool desc =
+        valid && E.mt < L.Up && ((E.d < (L.D - L.abs_err)) || (E.d * (1 + L.rel_err) < L.D));
+    const bool skip = valid && internal && !desc && hasch;
+    const int covered = excl_prefix_max(skip ? j + E.size : 0, lane);
+    const bool active = valid && j >= covered;
+    const bool miss = active && internal && desc && !hasch;
+    const bool tested = active && !internal && desc;
+    const bool improving = tested && E.dTri <= L.D;
+    const uint64_t stopm = wave_ballot(miss || improving);
+    const int stop = stopm ? (int)__builtin_ctzll(stopm) : n;
+    const bool in = active && j < stop;
+    const bool expand = in && internal && desc;
+    // pruned decisions lower mint (C2A.cpp:1284-1349, `else if`)
+    const double mc = wave_mind((in && !desc) ? E.mt : inf);
+    if (mc < L.M) L.M = mc;
+    const uint64_t em = wave_ballot(expand);
+    const int nexp = (int)__builtin_popcountll(em);
+    L.bv += 2 * nexp;
+    L.tri += (int)__builtin_popcountll(wave_ballot(in && tested));
+    // log the expansions in depth-first order (next step's replay)
+    if (!L.logfull && L.nlog + nexp > a.logcap) L.logfull = 1;
+    const int en = (expand && !L.logfull) ? L.nlog + lanes_below(em, lane) : -1;
+    const int pl = E.ppos - t0;
+    const int pen_in = lane_bcast(en, pl >= 0 ? pl : 0);
+    const int pen = E.ppos < 0 ? 0 : (pl >= 0 ? pen_in : g.anc[lev - 1 < 0 ? 0 : lev - 1]);
+    const int nn = (L.logfull || pen < 0) ? -1 : 2 * pen + 1 + ((E.rec - 1) & 1);
+    if (expand && en >= 0) {
+      LogEnt v;
+      v.rec = nn;
+      v.b1 = E.b1;
+      v.b2 = E.b2;
+      v.level = lev;
+      lw[en] = v;
+    }
+    if (!L.logfull) L.nlog += nexp;
+    L.nflush = L.nlog;
+    // the last decided event at each depth is the parent of later events
+    lane_sync();
+    if (in) lds_max(&g.lastlane[lev], j);
+    lane_sync();
+    const bool last = in && g.lastlane[lev] == j;
+    lane_sync();
+    if (last) {
+      g.anc[lev] = en;
+      g.lastlane[lev] = -1;
+    }
+    lane_sync();
+    if (stop < n) {
+      const double sd = lane_bcastd(E.d, stop), smt = lane_bcastd(E.mt, stop);
+      const double sdt = lane_bcastd(E.dTri, stop), sml = lane_bcastd(E.mleaf, stop);
+      const int sb1 = lane_bcast(E.b1, stop), sb2 = lane_bcast(E.b2, stop);
+      const int sinfo = lane_bcast(E.info, stop), srec = lane_bcast(E.rec, stop);
+      const int snn = lane_bcast(nn, stop);
+      L.spos = t0 + stop + 1;
+      if ((sinfo & 3) == 3) {
+        // improving leaf (C2A.cpp:1141-1183)
+        L.D = sdt;
+        L.pq_pending = 1;
+        if (sml <= L.M) L.M = sml;
+        L.last1 = sb1;
+        L.last2 = sb2;
+        L.tri++;
+        if (lane == 0) g.anc[sinfo >> 8] = -1;
+        lane_sync();
+        continue;
+      }
+      // a pair the previous step did not expand: sequential walk below it
+      Cur c;
+      c.d = sd;
+      c.mt = smt;
+      c.b1 = sb1;
+      c.b2 = sb2;
+      c.sg = 0;
+      c.flags = (sinfo & 3) | 8;
+      c.dTri = sdt;
+      c.mleaf = sml;
+      c.ar = srec;
+      c.cp = -1;
+      c.nn = snn;
+      c.lvl = sinfo >> 8;
+      if (lane == 0) g.anc[sinfo >> 8] = -1;
+      write_cur(g.stack[0], c, lane);
+      lane_sync();
+      L.sp = 1;
+      return 1;
+    }
+    const int reach = wave_maxi((in && skip) ? j + E.size : 0);
+    L.spos = t0 + (reach > n ? reach : n);
+  }
+  return 0;
 }
 
 // Stages replayed records [rec, rec + WIN) in LDS.
 template <int K, int CAP>
-C2A_HD void load_window(Wave<K, CAP>& g, const ArenaView& av, int rec, int nrec_max, int lane,
-                        int nl) {
-  lane_sync();
+C2A_HD void load_window(Wave<K, CAP>& g, Leader& L, const ArenaView& av, int rec, int nrec_max,
+                        int lane, int nl) {
   const HotRec C2A_GLOBAL* hot = gp(av.hot);
+  lane_sync();
   for (int i = lane; i < WIN; i += nl)
     if (rec + i < nrec_max) g.win[i] = hot[rec + i];
-  if (lane == 0) g.wbase = rec;
+  L.wbase = rec;
   lane_sync();
 }
 
-// The reference DFS as a stack machine (C2A.cpp:1113-1354 and, in contact
-// mode, 1525-1725).  Uniform across lanes.  Returns true when a round has
-// been requested.
+// Copies the staged log entries to the arena (lane-parallel).  Keeping
+// global stores out of the per-expansion path matters: a store's completion
+// is waited for wherever the walk next waits on vector memory.
+template <int K, int CAP>
+C2A_HD void flush_log(Wave<K, CAP>& g, Leader& L, const ArenaView& av, int lane, int nl) {
+  const int n = L.nlog - L.nflush;
+  if (n <= 0) return;
+  LogEnt C2A_GLOBAL* lw = gp(L.logbuf ? av.log1 : av.log0) + L.nflush;
+  lane_sync();
+  for (int i = lane; i < n; i += nl) lw[i] = g.lbuf[i];
+  L.nflush = L.nlog;
+  lane_sync();
+}
+
+// Puts the pair being processed back on the stack (a round will be rooted
+// at it).  where: 0 = its full entry is still at stack[sp] (it was just
+// popped), 1 = it is slot `csl` of the current round, 2 = a replayed record.
+template <int K, int CAP>
+C2A_HD void push_back(Wave<K, CAP>& g, Leader& L, const Cur& c, int where, int csl, int lane,
+                      int nl) {
+  if (where == 1) {
+    lane_sync();
+    slot_to_entry(g.stack[L.sp], g.slot[csl], c.sg, c.flags, c.nn, c.lvl, lane, nl);
+  } else if (where == 2) {
+    lane_sync();
+    write_cur(g.stack[L.sp], c, lane);
+  }
+  lane_sync();
+  L.sp++;
+}
+
+// The reference DFS (C2A.cpp:1113-1354 and, in contact mode, 1525-1725) as
+// a stack machine.  The pair being processed is held in registers; an
+// expansion pushes the child visited second and continues with the first,
+// so the stack holds exactly the reference's pending siblings.  Uniform
+// across lanes.  Returns true when a round has been requested.
 template <int K, int CAP>
 C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
   const bool contact = g.contact != 0;
-  const int gen = g.gen;
+  const int gen = uni(g.gen);
   const ArenaView av = arena_view(L.ab, a.logcap);
   const int nrec_max = 1 + 2 * a.logcap;
-  while (L.sp > 0) {
-    const Entry& e = g.stack[L.sp - 1];
-    const int flags = e.flags;
+  Cur c;
+  int where = 0, csl = 0;
+  bool have = false;
+  for (;;) {
+    if (!have) {
+      if (L.sp == 0) return false;
+      L.sp--;
+      c = read_cur(g.stack[L.sp]);
+      where = 0;
+    }
+    have = false;
+    const int flags = c.flags;
     if (!(flags & 4)) {
-      const double d = e.d, mt = e.mt;
       bool desc;
-      if (contact) desc = d < L.thr;   // abs = rel = 0 (C2A.cpp:1760-1761)
-      else desc = mt < L.Up && ((d < (L.D - L.abs_err)) || (d * (1 + L.rel_err) < L.D));
+      if (contact) desc = c.d < L.thr;   // abs = rel = 0 (C2A.cpp:1760-1761)
+      else desc = c.mt < L.Up && ((c.d < (L.D - L.abs_err)) || (c.d * (1 + L.rel_err) < L.D));
       if (!desc) {
-        if (!contact && mt < L.M) L.M = mt;
-        L.sp--;
+        if (!contact && c.mt < L.M) L.M = c.mt;
         continue;
       }
     }
-    const int sg = e.sg;
+    const int sg = c.sg;
     const bool cached = (sg >> 8) == gen && (sg & 0xff) != 0;
     const int s = (sg & 0xff) - 1;
     if ((flags & 3) == 3) {
       if (flags & 8) {
         // replayed leaf pair (C2A.cpp:1141-1183); p1/p2 are recomputed at the
         // end of the step if this stays the last improving leaf
-        const double dTri = e.dTri;
-        if (dTri <= L.D) {
-          L.D = dTri;
+        if (c.dTri <= L.D) {
+          L.D = c.dTri;
           L.pq_pending = 1;
-          if (e.mleaf <= L.M) L.M = e.mleaf;
-          L.last1 = e.b1;
-          L.last2 = e.b2;
+          if (c.mleaf <= L.M) L.M = c.mleaf;
+          L.last1 = c.b1;
+          L.last2 = c.b2;
         }
         L.tri++;
-        L.sp--;
         continue;
       }
       if (!cached) {
         C2A_EMU_COUNT(0, L);
+        push_back(g, L, c, where, csl, lane, nl);
         request_round(g, L, a, lane, nl);
         return true;
       }
       const Slot& sl = g.slot[s];
       if (!contact) {
-        const double dTri = sl.dTri;
+        const double dTri = unid(sl.dTri);
         if (dTri <= L.D) {
           L.D = dTri;
           L.pq_pending = 0;
-          for (int i = 0; i < 3; i++) { L.p1[i] = sl.p[i]; L.p2[i] = sl.q[i]; }
-          if (sl.mleaf <= L.M) L.M = sl.mleaf;
-          L.last1 = sl.t1;
-          L.last2 = sl.t2;
+          for (int i = 0; i < 3; i++) { L.p1[i] = unid(sl.p[i]); L.p2[i] = unid(sl.q[i]); }
+          const double ml = unid(sl.mleaf);
+          if (ml <= L.M) L.M = ml;
+          L.last1 = uni(sl.t1);
+          L.last2 = uni(sl.t2);
         }
         L.tri++;
       } else {
-        if ((sl.flags & 4) && sl.dTri <= L.thr) L.num_contact++;
+        if ((uni(sl.flags) & 4) && unid(sl.dTri) <= L.thr) L.num_contact++;
       }
-      L.sp--;
       continue;
     }
-    const int cp = e.cp;
-    const bool from_arena = (flags & 8) && cp >= 0;
+    const bool from_arena = (flags & 8) && c.cp >= 0;
     const int lvl = cached ? 31 - clz32((unsigned)(s + 1)) : K;
     if (!from_arena && (!cached || lvl >= K)) {
       C2A_EMU_COUNT((flags & 8) ? 1 : (cached ? 2 : 3), L);
+      push_back(g, L, c, where, csl, lane, nl);
       request_round(g, L, a, lane, nl);
       return true;
     }
-    if (!contact) L.bv += 2;
-    // log the expansion for the next step's replay
-    int nnA = -1;
-    const int enn = e.nn, elvl = e.lvl;
-    if (enn >= 0 && L.nlog < a.logcap) {
-      const int en = L.nlog++;
-      if (lane == 0) {
-        LogEnt C2A_GLOBAL* le = gp(L.logbuf ? av.log1 : av.log0) + en;
-        LogEnt v;
-        v.rec = enn;
-        v.b1 = e.b1;
-        v.b2 = e.b2;
-        v.level = elvl;
-        *le = v;
-      }
-      nnA = 2 * en + 1;
-    }
-    const int nnC = nnA < 0 ? -1 : nnA + 1;
-    L.sp--;
     if (L.sp + 2 > CAP) {
       L.flags |= C2A_FLAG_STACK_OVERFLOW;
       L.sp = 0;
       return false;
     }
-    if (from_arena) {
-      const int ra = 2 * cp + 1;
-      if (ra < g.wbase || ra + 1 >= g.wbase + WIN) load_window(g, av, ra, nrec_max, lane, nl);
-      const HotRec& A = g.win[ra - g.wbase];
-      const HotRec& C = g.win[ra + 1 - g.wbase];
-      // closer child visited first (C2A.cpp:1281): it goes on top
-      const bool c_first = C.d < A.d;
-      lane_sync();
-      if (c_first) {
-        hot_to_entry(g.stack[L.sp], A, ra, nnA, elvl + 1, lane, nl);
-        hot_to_entry(g.stack[L.sp + 1], C, ra + 1, nnC, elvl + 1, lane, nl);
-      } else {
-        hot_to_entry(g.stack[L.sp], C, ra + 1, nnC, elvl + 1, lane, nl);
-        hot_to_entry(g.stack[L.sp + 1], A, ra, nnA, elvl + 1, lane, nl);
+    if (!contact) L.bv += 2;
+    // log the expansion for the next step's replay
+    int nnA = -1;
+    if (c.nn >= 0 && L.nlog < a.logcap) {
+      const int en = L.nlog++;
+      if (lane == 0) {
+        LogEnt v;
+        v.rec = c.nn;
+        v.b1 = c.b1;
+        v.b2 = c.b2;
+        v.level = c.lvl;
+        g.lbuf[en - L.nflush] = v;
       }
+      nnA = 2 * en + 1;
+      if (L.nlog - L.nflush == LBUF) flush_log(g, L, av, lane, nl);
+    }
+    const int nnC = nnA < 0 ? -1 : nnA + 1;
+    const int clvl = c.lvl + 1;
+    if (from_arena) {
+      const int ra = 2 * c.cp + 1;
+      if (ra < L.wbase || ra + 1 >= L.wbase + WIN) load_window(g, L, av, ra, nrec_max, lane, nl);
+      const Cur cA = cur_from_hot(g.win[ra - L.wbase], ra, nnA, clvl);
+      const Cur cC = cur_from_hot(g.win[ra + 1 - L.wbase], ra + 1, nnC, clvl);
+      // closer child visited first (C2A.cpp:1281); the other one waits
+      const bool c_first = cC.d < cA.d;
       lane_sync();
-      L.sp += 2;
+      write_cur(g.stack[L.sp], c_first ? cA : cC, lane);
+      lane_sync();
+      L.sp++;
+      c = c_first ? cC : cA;
+      where = 2;
+      have = true;
       continue;
     }
     const int sa = 2 * s + 1, sc = 2 * s + 2;
-    const Slot& A = g.slot[sa];
-    const Slot& C = g.slot[sc];
-    const int sgA = (gen << 8) | (sa + 1), sgC = (gen << 8) | (sc + 1);
-    const int fA = A.flags & 3, fC = C.flags & 3;
-    const bool c_first = C.d < A.d;
+    const double dA = unid(g.slot[sa].d), dC = unid(g.slot[sc].d);
+    const bool c_first = dC < dA;
+    const int sf = c_first ? sc : sa, ss = c_first ? sa : sc;
+    const int nf = c_first ? nnC : nnA, ns = c_first ? nnA : nnC;
+    const Slot& F = g.slot[sf];
     lane_sync();
-    if (c_first) {
-      slot_to_entry(g.stack[L.sp], A, sgA, fA, nnA, elvl + 1, lane, nl);
-      slot_to_entry(g.stack[L.sp + 1], C, sgC, fC, nnC, elvl + 1, lane, nl);
-    } else {
-      slot_to_entry(g.stack[L.sp], C, sgC, fC, nnC, elvl + 1, lane, nl);
-      slot_to_entry(g.stack[L.sp + 1], A, sgA, fA, nnA, elvl + 1, lane, nl);
-    }
+    slot_to_entry(g.stack[L.sp], g.slot[ss], (gen << 8) | (ss + 1), uni(g.slot[ss].flags) & 3,
+                  ns, clvl, lane, nl);
     lane_sync();
-    L.sp += 2;
+    L.sp++;
+    c.d = c_first ? dC : dA;
+    c.mt = unid(F.mt);
+    c.b1 = uni(F.b1);
+    c.b2 = uni(F.b2);
+    c.sg = (gen << 8) | (sf + 1);
+    c.flags = uni(F.flags) & 3;
+    c.dTri = 0.0;
+    c.mleaf = 0.0;
+    c.ar = -1;
+    c.cp = -1;
+    c.nn = nf;
+    c.lvl = clvl;
+    where = 1;
+    csl = sf;
+    have = true;
   }
-  return false;
 }
 
 // After a step's walk: closest points of a replayed last improving leaf
 // (the same triangle-distance call a round would have made), and the log
 // hand-over to the next step.
 template <int K, int CAP>
-C2A_HD void end_walk(Wave<K, CAP>& g, Leader& L, const KArgs& a) {
+C2A_HD void end_walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
   if (L.pq_pending) {
     const MeshDev M1 = a.meshes[g.mesh[0]];
     const MeshDev M2 = a.meshes[g.mesh[1]];
@@ -704,6 +1082,7 @@ C2A_HD void end_walk(Wave<K, CAP>& g, Leader& L, const KArgs& a) {
     L.pq_pending = 0;
   }
   if (a.logcap > 0) {
+    flush_log(g, L, arena_view(L.ab, a.logcap), lane, nl);
     L.nlog_prev = L.nlog;
     L.logbuf ^= 1;
   }
@@ -723,6 +1102,7 @@ C2A_HD void begin_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   }
   start_traversal(g, L, M1, M2, R1, T1, R2, T2, a.logcap > 0 ? 0 : -1, lane);
   L.nlog = 0;
+  L.nflush = 0;
   L.pq_pending = 0;
   // initial bound from the caller-seeded pair (C2A.cpp:1812-1858)
   double p[3], q[3], ta[9], tb[9];
@@ -902,6 +1282,7 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     for (int i = 0; i < 3; i++) { T1[i] = q.T1[i]; T2[i] = q.T2[i]; }
     start_traversal(g, L, M1, M2, R1, T1, R2, T2, -1, lane);
     L.nlog = 0;
+    L.nflush = 0;
     L.pq_pending = 0;
     double p[3], qq[3], ta[9], tb[9], Rrel[9], Trel[3];
     gdouble_ptr g1 = tris_of(M1) + 9 * (int64_t)L.seed1;
@@ -978,12 +1359,35 @@ C2A_HD void leader_run(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int
         break;
       case ST_STEP:
         begin_step(g, L, a, lane);
-        if (a.logcap > 0 && L.nlog_prev > 0) replay(g, L, a, lane, nl);
-        L.state = ST_WALK;
+        if (a.logcap > 0 && L.nlog_prev > 0) {
+          replay(g, L, a, lane, nl);
+          start_scan(g, L, a, lane, nl);
+          L.state = ST_SCAN;
+        } else {
+          L.state = ST_WALK;
+        }
+        break;
+      case ST_SCAN:
+        if (scan(g, L, a, lane, nl)) {
+          L.state = ST_SUB;
+          break;
+        }
+        end_walk(g, L, a, lane, nl);
+        after_step(g, L, a, lane);
+        break;
+      case ST_SUB:
+        if (walk(g, L, a, lane, nl)) return;
+        flush_log(g, L, arena_view(L.ab, a.logcap), lane, nl);
+        if (L.flags & C2A_FLAG_STACK_OVERFLOW) {
+          end_walk(g, L, a, lane, nl);
+          after_step(g, L, a, lane);
+        } else {
+          L.state = ST_SCAN;
+        }
         break;
       case ST_WALK:
         if (walk(g, L, a, lane, nl)) return;
-        end_walk(g, L, a);
+        end_walk(g, L, a, lane, nl);
         after_step(g, L, a, lane);
         break;
       case ST_CONTACT_WALK:

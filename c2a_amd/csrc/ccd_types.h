@@ -68,6 +68,19 @@ struct HotRec {          // what the walk reads of a replayed record (64 B)
   int32_t pad[2];
 };
 
+// One replayed pair as the lane-parallel scan reads it: the records of the
+// previous step's subtree laid out in this step's depth-first order (64 B).
+struct Event {
+  double d, mt;          // RSS distance and admissible time
+  double dTri, mleaf;    // leaf pairs: triangle distance and leaf admissible time
+  int32_t b1, b2;        // node ids (leaf pairs: triangle indices)
+  int32_t rec;           // record number
+  int32_t size;          // events in its replayed subtree, itself included
+  int32_t ppos;          // scan position of its parent, -1 for the root
+  int32_t info;          // bit0/1 leaf nodes, bit2 has replayed children,
+                         // bit3 second child node, bits 8+ depth
+};
+
 // Per-wavefront replay storage, carved from one device allocation.
 struct ArenaView {
   LogEnt* log0;          // written by the walk / read by the next replay, alternating
@@ -76,13 +89,17 @@ struct ArenaView {
   HotRec* hot;           // nrec
   int32_t* cpa;          // nrec: expansion index of each record's children
   int32_t* order;        // logcap: expansions sorted by level
+  int32_t* sz;           // nrec: replayed subtree size of each record
+  int32_t* pos;          // nrec: scan position of each record
+  Event* ev;             // nrec - 1: the scan order
 };
 
 C2A_HD size_t arena_bytes(int logcap) {
   const size_t nrec = 1 + 2 * (size_t)logcap;
   size_t b = 2 * (size_t)logcap * sizeof(LogEnt) + nrec * 12 * sizeof(double) +
-             nrec * sizeof(HotRec) + nrec * sizeof(int32_t) + (size_t)logcap * sizeof(int32_t);
-  return (b + 255) & ~(size_t)255;
+             nrec * sizeof(HotRec) + nrec * sizeof(int32_t) + (size_t)logcap * sizeof(int32_t) +
+             2 * nrec * sizeof(int32_t) + nrec * sizeof(Event);
+  return (b + 16 + 255) & ~(size_t)255;
 }
 
 C2A_HD ArenaView arena_view(unsigned char* base, int logcap) {
@@ -100,6 +117,13 @@ C2A_HD ArenaView arena_view(unsigned char* base, int logcap) {
   v.cpa = (int32_t*)p;
   p += nrec * sizeof(int32_t);
   v.order = (int32_t*)p;
+  p += (size_t)logcap * sizeof(int32_t);
+  v.sz = (int32_t*)p;
+  p += nrec * sizeof(int32_t);
+  v.pos = (int32_t*)p;
+  p += nrec * sizeof(int32_t);
+  p = (unsigned char*)(((uintptr_t)p + 15) & ~(uintptr_t)15);
+  v.ev = (Event*)p;
   return v;
 }
 
