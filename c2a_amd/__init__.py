@@ -33,7 +33,7 @@ class Params(C.Structure):
     _fields_ = [("tol_d", C.c_double), ("tol_t", C.c_double), ("max_iter", C.c_int32),
                 ("with_contacts", C.c_int32), ("contact_scale", C.c_double),
                 ("contact_offset", C.c_double), ("spec_depth", C.c_int32),
-                ("groups_per_wave", C.c_int32), ("replay_log", C.c_int32), ("pad", C.c_int32)]
+                ("waves_per_query", C.c_int32), ("replay_log", C.c_int32), ("pad", C.c_int32)]
 
 
 class Result(C.Structure):

@@ -84,7 +84,7 @@ struct c2a_batch {
   DevBuf d_mesh, d_q, d_out, d_counter, d_arena;
   int64_t nq = 0;
   KArgs args{};
-  int K = 5, CAP = 48, blocks = 1;
+  int K = 5, CAP = 48, NW = 4, blocks = 1;
   int logcap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
@@ -135,28 +135,26 @@ int pick_cap(const std::vector<MeshDev>& tab, const int32_t* pairs, int64_t n, i
   return -1;
 }
 
-int pick_shape(const c2a_params* p, int& K) {
+int pick_shape(const c2a_params* p, int& K, int& NW) {
   K = (p && p->spec_depth > 0) ? p->spec_depth : 5;
+  NW = (p && p->waves_per_query > 0) ? p->waves_per_query : 1;
   if (K < 1 || K > 6) return fail(C2A_ERR_BAD_ARG, "spec_depth must be 1..6");
+  if (!kernel_shape_ok(K, 48, NW)) return fail(C2A_ERR_BAD_ARG, "waves_per_query must be 0 or 1");
   return C2A_OK;
 }
 
-int grid_blocks(int K, int CAP, int64_t nq) {
+// Resident workgroups: the kernel is persistent, so launching more than fit
+// at once only adds workgroups that find the queue empty.
+int grid_blocks(int K, int CAP, int NW, int64_t nq) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   hipDeviceProp_t prop;
   int cus = 256;
-  size_t lds_cu = 160 * 1024;
-  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
-    cus = prop.multiProcessorCount;
-    if (prop.maxSharedMemoryPerMultiProcessor > 0) lds_cu = prop.maxSharedMemoryPerMultiProcessor;
-  }
-  size_t lds = kernel_lds_bytes(K, CAP);
-  int per_cu = lds ? (int)std::max<size_t>(1, lds_cu / lds) : 1;
-  per_cu = std::min(per_cu, 16);
-  int64_t want = nq;
-  int64_t cap = (int64_t)cus * per_cu;
-  return (int)std::max<int64_t>(1, std::min(want, cap));
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+  int per_cu = kernel_blocks_per_cu(K, CAP, NW);
+  if (per_cu < 1) per_cu = 1;
+  const int64_t cap = (int64_t)cus * per_cu;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(nq, cap));
 }
 
 int prepare_queries(const std::vector<MeshDev>& tab, const c2a_query* q, int64_t nq,
@@ -208,11 +206,11 @@ int fill_args(c2a_batch* b, const c2a_params* p) {
 
 __global__ void zero_counter(int64_t* c) { *c = 0; }
 
-int run_kernel(const KArgs& a, int K, int CAP, int blocks, hipStream_t s) {
+int run_kernel(const KArgs& a, int K, int CAP, int NW, int blocks, hipStream_t s) {
   // counter cleared by a kernel (graph-capture safe, no memset node)
   hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, a.counter);
   HIP_TRY(hipGetLastError());
-  hipError_t e = launch_ccd(a, K, CAP, blocks, s);
+  hipError_t e = launch_ccd(a, K, CAP, NW, blocks, s);
   if (e != hipSuccess) return hip_fail(e, "launch_ccd");
   return C2A_OK;
 }
@@ -231,7 +229,7 @@ void c2a_params_default(c2a_params* p) {
   p->contact_scale = 2.0;
   p->contact_offset = 0.001;
   p->spec_depth = 0;
-  p->groups_per_wave = 0;
+  p->waves_per_query = 0;
   p->replay_log = 8192;
   p->pad = 0;
 }
@@ -323,7 +321,7 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     return fail(C2A_ERR_OOM, "host allocation failed");
   }
   b->nq = nq;
-  if ((rc = pick_shape(params, b->K))) {
+  if ((rc = pick_shape(params, b->K, b->NW))) {
     delete b;
     return rc;
   }
@@ -332,7 +330,7 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     delete b;
     return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
   }
-  b->blocks = grid_blocks(b->K, b->CAP, nq);
+  b->blocks = grid_blocks(b->K, b->CAP, b->NW, nq);
   {
     c2a_params def;
     c2a_params_default(&def);
@@ -372,7 +370,7 @@ int c2a_batch_run(c2a_batch* b, void* stream) {
   if (b->nq == 0) return C2A_OK;
   hipStream_t s = (hipStream_t)stream;
   HIP_TRY(hipEventRecord(b->ev0, s));
-  int rc = run_kernel(b->args, b->K, b->CAP, b->blocks, s);
+  int rc = run_kernel(b->args, b->K, b->CAP, b->NW, b->blocks, s);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(b->ev1, s));
   b->timed = true;
@@ -469,7 +467,7 @@ int c2a_ccd_step(c2a_mesh* const* meshes, int32_t nmesh, const c2a_step_query* q
   a.nq = nq;
   a.mode = MODE_STEP;
   a.max_iter = 150;
-  if ((rc = run_kernel(a, K, CAP, grid_blocks(K, CAP, nq), s))) return rc;
+  if ((rc = run_kernel(a, K, CAP, 1, grid_blocks(K, CAP, 1, nq), s))) return rc;
   HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)nq * sizeof(c2a_step_result), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return C2A_OK;
@@ -517,7 +515,7 @@ int c2a_contact_count(c2a_mesh* const* meshes, int32_t nmesh, const double* pose
   a.counter = (int64_t*)dc.p;
   a.nq = nq;
   a.mode = MODE_CONTACT;
-  if ((rc = run_kernel(a, K, CAP, grid_blocks(K, CAP, nq), s))) return rc;
+  if ((rc = run_kernel(a, K, CAP, 1, grid_blocks(K, CAP, 1, nq), s))) return rc;
   HIP_TRY(hipMemcpyAsync(counts_out, dout.p, (size_t)nq * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return C2A_OK;

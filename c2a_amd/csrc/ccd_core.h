@@ -30,6 +30,10 @@
 #ifndef C2A_EMU_COUNT
 #define C2A_EMU_COUNT(kind, L) ((void)0)
 #endif
+// Index checks compiled only into the host-side test harness.
+#ifndef C2A_CHECK
+#define C2A_CHECK(cond) ((void)0)
+#endif
 
 namespace c2a {
 
@@ -41,14 +45,15 @@ C2A_HD int clz32(unsigned x) {
 #endif
 }
 
-// Makes the LDS writes of all lanes visible to all lanes.  A workgroup is
-// one wavefront, whose LDS accesses complete in issue order, so this only has
-// to keep the compiler from reordering LDS accesses across it and drain the
-// LDS counter; outstanding global stores are not waited for.
+// Makes the LDS writes of all threads of the workgroup visible to all of
+// them: drain the LDS counter, then a workgroup barrier.  LDS is coherent
+// within the CU, so no cache maintenance is needed, and outstanding global
+// stores are not waited for (a full __syncthreads would wait for them).
 C2A_HD void lane_sync() {
 #if defined(__HIP_DEVICE_COMPILE__)
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+  __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #endif
 }
@@ -136,6 +141,7 @@ struct Cur {
 };
 
 C2A_HD Cur read_cur(const Entry& e) {
+  C2A_CHECK(e.flags >= 0);
   Cur c = *reinterpret_cast<const Cur*>(&e.d);
   c.d = unid(c.d);
   c.mt = unid(c.mt);
@@ -212,6 +218,10 @@ struct Wave {
   LogEnt lbuf[LBUF];         // log entries [L.nflush, L.nlog) not yet in the arena
   int32_t anc[CAP + 2];      // scan: log index of the last event at each depth
   int32_t lastlane[CAP + 2]; // scan: scratch for the per-depth last event
+  int32_t nleafq;            // replay: leaf records queued for the triangle pass
+  int64_t ticket;            // work-queue ticket shared by the wavefronts
+  int32_t rarena;            // the pending round writes its subtree to the arena
+  int32_t rpb;               // ... as pairs [rpb, rpb + 2^K - 1)
   int32_t lh[CAP + 2];       // replay: expansions per level, then fill cursors
   int32_t ls[CAP + 3];       // replay: first sorted expansion of each level
 };
@@ -245,6 +255,7 @@ struct Leader {
   int pq_pending;                // p1, p2 of the last improving leaf still to compute
   int wbase;                     // first record staged in the LDS window
   int spos, nev, logfull;        // scan position, events, log capacity reached
+  int napp;                      // next free record pair of the arena
   unsigned char* ab;             // this wavefront's replay arena
 };
 
@@ -305,6 +316,7 @@ C2A_HD void leaf_values(Slot& s, const MeshDev& M1, const MeshDev& M2, const dou
 // child pair `bit` (0 = the first child node, 1 = its sibling).  Returns
 // false, changing nothing, for a leaf pair.
 C2A_HD bool descend(node_ptr N1, node_ptr N2, int& b1, int& b2, int bit, double* R, double* T) {
+  C2A_CHECK(b1 >= 0 && b2 >= 0);
   node_ptr n1 = N1 + b1;
   node_ptr n2 = N2 + b2;
   const int fc1 = n1->first_child, fc2 = n2->first_child;
@@ -367,8 +379,8 @@ C2A_HD int pair_values(node_ptr N1, node_ptr N2, int b1, int b2, const double* R
 // Heap slot h of the depth-K subtree below X: the pair reached by the path
 // bits of h+1, its transform chained from X, and its values.
 template <int K, int CAP>
-C2A_HD void compute_slot(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
-                         bool contact) {
+C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
+                              bool contact, Slot& s) {
   node_ptr N1 = nodes_of(M1);
   node_ptr N2 = nodes_of(M2);
   const int lvl = 31 - clz32((unsigned)(h + 1));
@@ -376,7 +388,6 @@ C2A_HD void compute_slot(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDe
   for (int i = 0; i < 9; i++) R[i] = g.X.R[i];
   for (int i = 0; i < 3; i++) T[i] = g.X.T[i];
   int b1 = g.X.b1, b2 = g.X.b2;
-  Slot& s = g.slot[h];
   for (int lv = 0; lv < lvl; lv++) {
     const int bit = ((h + 1) >> (lvl - 1 - lv)) & 1;
     if (!descend(N1, N2, b1, b2, bit, R, T)) {
@@ -385,6 +396,7 @@ C2A_HD void compute_slot(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDe
     }
   }
   double d, mt;
+  C2A_CHECK(b1 >= 0 && b2 >= 0 && b1 < 2 * M1.ntri - 1 && b2 < 2 * M2.ntri - 1);
   const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], contact, d, mt);
   for (int i = 0; i < 9; i++) s.R[i] = R[i];
   for (int i = 0; i < 3; i++) s.T[i] = T[i];
@@ -398,9 +410,47 @@ C2A_HD void compute_slot(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDe
     leaf_values(s, M1, M2, g.Rrel, g.Trel, g.r1, g.tt1, g.mo[0], g.mo[1], b1, b2, contact);
 }
 
+template <int K, int CAP>
+C2A_HD void compute_slot(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
+                         bool contact) {
+  compute_slot_into(g, h, M1, M2, contact, g.slot[h]);
+}
+
+// Heap slot h of the round written as arena record 2 * rpb + h, so that the
+// pair stays available for the rest of the step; slots above the last level
+// link their children (pair rpb + h).
+template <int K, int CAP>
+C2A_HD void compute_slot_arena(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
+                               const ArenaView& av) {
+  Slot s;
+  compute_slot_into(g, h, M1, M2, false, s);
+  if (!s.valid) return;   // below a leaf pair: never referenced
+  const int rpb = g.rpb;
+  const int rec = 2 * rpb + h;
+  C2A_CHECK(rec > 0);
+  const int lvl = 31 - clz32((unsigned)(h + 1));
+  double C2A_GLOBAL* xf = gp(av.xf) + 12 * (int64_t)rec;
+  for (int i = 0; i < 9; i++) xf[i] = s.R[i];
+  for (int i = 0; i < 3; i++) xf[9 + i] = s.T[i];
+  HotRec hr;
+  const bool leaf = (s.flags & 3) == 3;
+  hr.d = s.d;
+  hr.mt = s.mt;
+  hr.dTri = leaf ? s.dTri : 0.0;
+  hr.mleaf = leaf ? s.mleaf : 0.0;
+  hr.b1 = s.b1;
+  hr.b2 = s.b2;
+  hr.flags = s.flags & 3;
+  hr.cp = (!leaf && lvl < K) ? rpb + h : -1;
+  hr.t1 = leaf ? s.t1 : -1;
+  hr.t2 = leaf ? s.t2 : -1;
+  hr.pad[0] = hr.pad[1] = 0;
+  gp(av.hot)[rec] = hr;
+}
+
 // One lane's share of a round.
 template <int K, int CAP>
-C2A_HD void round_lane(Wave<K, CAP>& g, int lane, int nl, const KArgs& a) {
+C2A_HD void round_lane(Wave<K, CAP>& g, const Leader& L, int lane, int nl, const KArgs& a) {
   const MeshDev M1 = a.meshes[g.mesh[0]];
   const MeshDev M2 = a.meshes[g.mesh[1]];
   const bool contact = g.contact != 0;
@@ -414,6 +464,11 @@ C2A_HD void round_lane(Wave<K, CAP>& g, int lane, int nl, const KArgs& a) {
       leaf_values(s, M1, M2, g.Rrel, g.Trel, g.r1, g.tt1, g.mo[0], g.mo[1], g.X.b1, g.X.b2,
                   contact);
     }
+    return;
+  }
+  if (g.rarena) {
+    const ArenaView av = arena_view(L.ab, a.logcap);
+    for (int h = lane + 1; h < Shape<K>::NSLOT; h += nl) compute_slot_arena(g, h, M1, M2, av);
     return;
   }
   for (int h = lane + 1; h < Shape<K>::NSLOT; h += nl) compute_slot(g, h, M1, M2, contact);
@@ -434,6 +489,7 @@ C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, 
   const uint64_t* s = reinterpret_cast<const uint64_t*>(&e);
   uint64_t* x = reinterpret_cast<uint64_t*>(&g.X);
   if (flags & 8) {
+    C2A_CHECK(ar >= 0 && ar < arena_records(a.logcap));
     // replayed pair: the transform is in the arena
     const uint64_t C2A_GLOBAL* xf = gp(reinterpret_cast<const uint64_t*>(
         arena_view(L.ab, a.logcap).xf + 12 * (int64_t)ar));
@@ -442,16 +498,32 @@ C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, 
     for (int w = lane; w < 16; w += nl) x[w] = s[w];
   }
   lane_sync();
+  // internal pairs of a CA step go to the arena while it has room
+  constexpr int NPAIR = (1 << K) - 1;
+  const bool to_arena = a.arena_rounds && a.logcap > 0 && !g.contact && (flags & 3) != 3 &&
+                        L.napp + NPAIR <= 2 * a.logcap;
   if (lane == 0) {
-    e.sg = sg;
-    Slot& s0 = g.slot[0];
-    s0.b1 = g.X.b1;
-    s0.b2 = g.X.b2;
-    s0.d = g.X.d;
-    s0.mt = g.X.mt;
-    s0.flags = flags & 3;
-    s0.valid = 1;
-    g.gen = gen;
+    if (to_arena) {
+      g.rarena = 1;
+      g.rpb = L.napp;
+      e.cp = L.napp;
+      e.flags = flags | 8;
+    } else {
+      g.rarena = 0;
+      e.sg = sg;
+      Slot& s0 = g.slot[0];
+      s0.b1 = g.X.b1;
+      s0.b2 = g.X.b2;
+      s0.d = g.X.d;
+      s0.mt = g.X.mt;
+      s0.flags = flags & 3;
+      s0.valid = 1;
+      g.gen = gen;
+    }
+  }
+  if (to_arena) {
+    L.napp += NPAIR;
+    L.wbase = -(1 << 30);   // the window may hold stale copies of these records
   }
   lane_sync();
   L.need_round = 1;
@@ -522,7 +594,9 @@ C2A_HD void replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl)
   node_ptr N2 = nodes_of(M2);
   mem_sync();   // the previous walk's log stores
   // level histogram, children links
+  int32_t C2A_GLOBAL* leafq = gp(av.pos);   // free until the layout pass
   for (int i = lane; i < CAP + 2; i += nl) g.lh[i] = 0;
+  if (lane == 0) g.nleafq = 0;
   for (int i = lane; i < nrec; i += nl) cpa[i] = -1;
   mem_sync();
   for (int e = lane; e < n; e += nl) lds_add(&g.lh[lg[e].level], 1);
@@ -580,19 +654,25 @@ C2A_HD void replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl)
       h.t1 = -1;
       h.t2 = -1;
       h.pad[0] = h.pad[1] = 0;
-      if (fl == 3) {
-        Slot tmp;
-        tmp.flags = 3;
-        leaf_values(tmp, M1, M2, g.Rrel, g.Trel, g.r1, g.tt1, g.mo[0], g.mo[1], b1, b2, false);
-        h.dTri = tmp.dTri;
-        h.mleaf = tmp.mleaf;
-        h.t1 = tmp.t1;
-        h.t2 = tmp.t2;
-      }
       hot[rec] = h;
+      if (fl == 3) leafq[lds_add(&g.nleafq, 1)] = rec;
     }
     mem_sync();
   }
+  // triangle distances of the leaf pairs, in dense chunks (C2A.cpp:1141-1166)
+  const int nleaf = g.nleafq;
+  for (int k = lane; k < nleaf; k += nl) {
+    const int rec = leafq[k];
+    HotRec C2A_GLOBAL* hr = hot + rec;
+    Slot tmp;
+    tmp.flags = 3;
+    leaf_values(tmp, M1, M2, g.Rrel, g.Trel, g.r1, g.tt1, g.mo[0], g.mo[1], hr->b1, hr->b2, false);
+    hr->dTri = tmp.dTri;
+    hr->mleaf = tmp.mleaf;
+    hr->t1 = tmp.t1;
+    hr->t2 = tmp.t2;
+  }
+  mem_sync();
   // Scan layout: replayed subtree sizes bottom-up, then depth-first
   // positions top-down in this step's visiting order (closer child first,
   // C2A.cpp:1281), then the records scattered into that order.
@@ -764,10 +844,13 @@ C2A_HD int scan(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
   const Event C2A_GLOBAL* ev = gp(av.ev);
   LogEnt C2A_GLOBAL* lw = gp(L.logbuf ? av.log1 : av.log0);
   const double inf = 1.0 / 0.0;
+  // every wavefront of the workgroup decides the same 64 events (its lane j
+  // takes event t0 + j) and so holds the same state
+  const int cw = nl < 64 ? nl : 64;
+  const int j = lane & 63;
   while (L.spos < L.nev) {
     const int t0 = L.spos;
-    const int n = (L.nev - t0) < nl ? (L.nev - t0) : nl;
-    const int j = lane;
+    const int n = (L.nev - t0) < cw ? (L.nev - t0) : cw;
     const bool valid = j < n;
     Event E;
     if (valid) {
@@ -779,12 +862,13 @@ C2A_HD int scan(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
       E.info = 3;
     }
     const int lev = E.info >> 8;
+    C2A_CHECK(lev >= 1 && lev < CAP + 2 && E.size >= 1);
     const bool internal = (E.info & 3) != 3;
     const bool hasch = (E.info & 4) != 0;
     const bool desc =
         valid && E.mt < L.Up && ((E.d < (L.D - L.abs_err)) || (E.d * (1 + L.rel_err) < L.D));
     const bool skip = valid && internal && !desc && hasch;
-    const int covered = excl_prefix_max(skip ? j + E.size : 0, lane);
+    const int covered = excl_prefix_max(skip ? j + E.size : 0, j);
     const bool active = valid && j >= covered;
     const bool miss = active && internal && desc && !hasch;
     const bool tested = active && !internal && desc;
@@ -802,12 +886,12 @@ C2A_HD int scan(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
     L.tri += (int)__builtin_popcountll(wave_ballot(in && tested));
     // log the expansions in depth-first order (next step's replay)
     if (!L.logfull && L.nlog + nexp > a.logcap) L.logfull = 1;
-    const int en = (expand && !L.logfull) ? L.nlog + lanes_below(em, lane) : -1;
+    const int en = (expand && !L.logfull) ? L.nlog + lanes_below(em, j) : -1;
     const int pl = E.ppos - t0;
     const int pen_in = lane_bcast(en, pl >= 0 ? pl : 0);
     const int pen = E.ppos < 0 ? 0 : (pl >= 0 ? pen_in : g.anc[lev - 1 < 0 ? 0 : lev - 1]);
     const int nn = (L.logfull || pen < 0) ? -1 : 2 * pen + 1 + ((E.rec - 1) & 1);
-    if (expand && en >= 0) {
+    if (expand && en >= 0 && lane < 64) {
       LogEnt v;
       v.rec = nn;
       v.b1 = E.b1;
@@ -878,6 +962,7 @@ template <int K, int CAP>
 C2A_HD void load_window(Wave<K, CAP>& g, Leader& L, const ArenaView& av, int rec, int nrec_max,
                         int lane, int nl) {
   const HotRec C2A_GLOBAL* hot = gp(av.hot);
+  C2A_CHECK(rec >= 0 && rec + 1 < nrec_max);
   lane_sync();
   for (int i = lane; i < WIN; i += nl)
     if (rec + i < nrec_max) g.win[i] = hot[rec + i];
@@ -926,7 +1011,7 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
   const bool contact = g.contact != 0;
   const int gen = uni(g.gen);
   const ArenaView av = arena_view(L.ab, a.logcap);
-  const int nrec_max = 1 + 2 * a.logcap;
+  const int nrec_max = arena_records(a.logcap);
   Cur c;
   int where = 0, csl = 0;
   bool have = false;
@@ -1023,6 +1108,7 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
     if (from_arena) {
       const int ra = 2 * c.cp + 1;
       if (ra < L.wbase || ra + 1 >= L.wbase + WIN) load_window(g, L, av, ra, nrec_max, lane, nl);
+      C2A_CHECK(ra - L.wbase >= 0 && ra + 1 - L.wbase < WIN && L.sp < CAP);
       const Cur cA = cur_from_hot(g.win[ra - L.wbase], ra, nnA, clvl);
       const Cur cC = cur_from_hot(g.win[ra + 1 - L.wbase], ra + 1, nnC, clvl);
       // closer child visited first (C2A.cpp:1281); the other one waits
@@ -1104,6 +1190,8 @@ C2A_HD void begin_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   L.nlog = 0;
   L.nflush = 0;
   L.pq_pending = 0;
+  L.napp = a.logcap > 0 ? L.nlog_prev : 0;   // round records follow the replayed ones
+  L.wbase = -(1 << 30);
   // initial bound from the caller-seeded pair (C2A.cpp:1812-1858)
   double p[3], q[3], ta[9], tb[9];
   gdouble_ptr g1 = tris_of(M1) + 9 * (int64_t)L.seed1;
@@ -1231,21 +1319,23 @@ C2A_HD void after_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   finish_query(g, L, a, false, lane);
 }
 
-// Takes the next query from the work queue (one atomic per wavefront).
-C2A_HD int64_t take_ticket(int64_t* counter, int lane) {
+// Takes the next query from the work queue (one atomic per workgroup).
+template <int K, int CAP>
+C2A_HD int64_t take_ticket(Wave<K, CAP>& g, int64_t* counter, int lane) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  long long t = 0;
-  if (lane == 0) t = (long long)atomicAdd((unsigned long long*)counter, 1ull);
-  return (int64_t)__shfl(t, 0);
+  if (lane == 0) g.ticket = (int64_t)atomicAdd((unsigned long long*)counter, 1ull);
 #else
-  (void)lane;
-  return (*counter)++;
+  if (lane == 0) g.ticket = (*counter)++;
 #endif
+  lane_sync();
+  const int64_t t = g.ticket;
+  lane_sync();
+  return t;
 }
 
 template <int K, int CAP>
 C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
-  int64_t qi = take_ticket(a.counter, lane);
+  int64_t qi = take_ticket(g, a.counter, lane);
   if (qi >= a.nq) {
     L.state = ST_DONE;
     return false;

@@ -1,10 +1,11 @@
 // ccd_kernel.hip -- batched conservative-advancement CCD on gfx950.
 //
-// One wavefront (one 64-lane workgroup) per query.  The kernel is persistent:
-// wavefronts take queries from a device work queue until it drains.  Each
-// loop iteration advances the query's walk in lockstep (ccd_core.h) until it
-// needs pair values, then every lane computes its slots of the speculative
-// subtree.  All arithmetic is FP64 in the reference's operation
+// One workgroup of NW wavefronts per query.  The kernel is persistent:
+// workgroups take queries from a device work queue until it drains.  Each
+// loop iteration advances the query's walk (ccd_core.h; every wavefront
+// makes the same decisions) until it needs pair values, then every thread
+// computes its slots of the speculative subtree; the replay of a CA step
+// uses all threads of the workgroup.  All arithmetic is FP64 in the reference's operation
 // order, so results match the host oracle bit for bit.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -16,8 +17,8 @@
 
 namespace c2a {
 
-template <int K, int CAP>
-__global__ void __launch_bounds__(64) ccd_kernel(KArgs a) {
+template <int K, int CAP, int NW>
+__global__ void __launch_bounds__(64 * NW) ccd_kernel(KArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Wave<K, CAP>& g = *reinterpret_cast<Wave<K, CAP>*>(smem_raw);
   const int lane = threadIdx.x;
@@ -26,45 +27,76 @@ __global__ void __launch_bounds__(64) ccd_kernel(KArgs a) {
   L.sp = 0;
   L.ab = a.logcap > 0 ? a.arena + (size_t)blockIdx.x * arena_bytes(a.logcap) : nullptr;
   for (;;) {
-    leader_run(g, L, a, lane, 64);
+    leader_run(g, L, a, lane, 64 * NW);
     if (L.state == ST_DONE) break;
-    round_lane(g, lane, 64, a);
-    lane_sync();
+    round_lane(g, L, lane, 64 * NW, a);
+    if (g.rarena) mem_sync();   // the walk reads the new records through the LDS window
+    else lane_sync();
   }
 }
 
 // ---------------------------------------------------------------------------
 // host-side launch
 // ---------------------------------------------------------------------------
-template <int K, int CAP>
+template <int K, int CAP, int NW>
 static hipError_t launch_one(const KArgs& a, int blocks, hipStream_t s) {
   const size_t lds = sizeof(Wave<K, CAP>);
-  auto fn = ccd_kernel<K, CAP>;
+  auto fn = ccd_kernel<K, CAP, NW>;
   hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), lds, s, a);
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(64 * NW), lds, s, a);
   return hipGetLastError();
 }
 
+template <int K, int CAP, int NW>
+static int occupancy_one() {
+  const size_t lds = sizeof(Wave<K, CAP>);
+  auto fn = ccd_kernel<K, CAP, NW>;
+  if (hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+      hipSuccess)
+    return 0;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * NW, lds) != hipSuccess) return 0;
+  return n;
+}
+
+// Instantiated shapes: every K with one wavefront per query.  The kernel
+// supports NW > 1 (the replay then uses every wavefront), but a second
+// instantiation of the same K/CAP makes the compiler outline the shared
+// per-query code, which costs more than the extra wavefronts gain.
+#define C2A_SHAPES(X)                                                                  \
+  X(1, 48, 1) X(2, 48, 1) X(3, 48, 1) X(4, 48, 1) X(5, 48, 1) X(6, 48, 1)              \
+  X(1, 96, 1) X(2, 96, 1) X(3, 96, 1) X(4, 96, 1) X(5, 96, 1) X(6, 96, 1)
+
+bool kernel_shape_ok(int K, int CAP, int NW) {
+#define C2A_OK_CASE(k, cap, nw) \
+  if (K == k && CAP == cap && NW == nw) return true;
+  C2A_SHAPES(C2A_OK_CASE)
+#undef C2A_OK_CASE
+  return false;
+}
+
 size_t kernel_lds_bytes(int K, int CAP) {
-#define C2A_LDS_CASE(k, cap) \
+#define C2A_LDS_CASE(k, cap, nw) \
   if (K == k && CAP == cap) return sizeof(Wave<k, cap>);
-  C2A_LDS_CASE(1, 48) C2A_LDS_CASE(2, 48) C2A_LDS_CASE(3, 48) C2A_LDS_CASE(4, 48)
-  C2A_LDS_CASE(5, 48) C2A_LDS_CASE(6, 48)
-  C2A_LDS_CASE(1, 96) C2A_LDS_CASE(2, 96) C2A_LDS_CASE(3, 96) C2A_LDS_CASE(4, 96)
-  C2A_LDS_CASE(5, 96) C2A_LDS_CASE(6, 96)
+  C2A_SHAPES(C2A_LDS_CASE)
 #undef C2A_LDS_CASE
   return 0;
 }
 
-hipError_t launch_ccd(const KArgs& a, int K, int CAP, int blocks, hipStream_t s) {
-#define C2A_LAUNCH_CASE(k, cap) \
-  if (K == k && CAP == cap) return launch_one<k, cap>(a, blocks, s);
-  C2A_LAUNCH_CASE(1, 48) C2A_LAUNCH_CASE(2, 48) C2A_LAUNCH_CASE(3, 48) C2A_LAUNCH_CASE(4, 48)
-  C2A_LAUNCH_CASE(5, 48) C2A_LAUNCH_CASE(6, 48)
-  C2A_LAUNCH_CASE(1, 96) C2A_LAUNCH_CASE(2, 96) C2A_LAUNCH_CASE(3, 96) C2A_LAUNCH_CASE(4, 96)
-  C2A_LAUNCH_CASE(5, 96) C2A_LAUNCH_CASE(6, 96)
+int kernel_blocks_per_cu(int K, int CAP, int NW) {
+#define C2A_OCC_CASE(k, cap, nw) \
+  if (K == k && CAP == cap && NW == nw) return occupancy_one<k, cap, nw>();
+  C2A_SHAPES(C2A_OCC_CASE)
+#undef C2A_OCC_CASE
+  return 0;
+}
+
+hipError_t launch_ccd(const KArgs& a, int K, int CAP, int NW, int blocks, hipStream_t s) {
+#define C2A_LAUNCH_CASE(k, cap, nw) \
+  if (K == k && CAP == cap && NW == nw) return launch_one<k, cap, nw>(a, blocks, s);
+  C2A_SHAPES(C2A_LAUNCH_CASE)
 #undef C2A_LAUNCH_CASE
   return hipErrorInvalidValue;
 }

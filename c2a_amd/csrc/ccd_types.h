@@ -94,16 +94,22 @@ struct ArenaView {
   Event* ev;             // nrec - 1: the scan order
 };
 
+// Records: the replayed ones [0, 1 + 2 * logcap), then room for as many
+// again computed by speculative rounds during the step.
+C2A_HD int arena_records(int logcap) { return 1 + 4 * logcap; }
+
 C2A_HD size_t arena_bytes(int logcap) {
   const size_t nrec = 1 + 2 * (size_t)logcap;
-  size_t b = 2 * (size_t)logcap * sizeof(LogEnt) + nrec * 12 * sizeof(double) +
-             nrec * sizeof(HotRec) + nrec * sizeof(int32_t) + (size_t)logcap * sizeof(int32_t) +
+  const size_t nall = (size_t)arena_records(logcap);
+  size_t b = 2 * (size_t)logcap * sizeof(LogEnt) + nall * 12 * sizeof(double) +
+             nall * sizeof(HotRec) + nrec * sizeof(int32_t) + (size_t)logcap * sizeof(int32_t) +
              2 * nrec * sizeof(int32_t) + nrec * sizeof(Event);
   return (b + 16 + 255) & ~(size_t)255;
 }
 
 C2A_HD ArenaView arena_view(unsigned char* base, int logcap) {
   const size_t nrec = 1 + 2 * (size_t)logcap;
+  const size_t nall = (size_t)arena_records(logcap);
   ArenaView v;
   unsigned char* p = base;
   v.log0 = (LogEnt*)p;
@@ -111,9 +117,9 @@ C2A_HD ArenaView arena_view(unsigned char* base, int logcap) {
   v.log1 = (LogEnt*)p;
   p += (size_t)logcap * sizeof(LogEnt);
   v.xf = (double*)p;
-  p += nrec * 12 * sizeof(double);
+  p += nall * 12 * sizeof(double);
   v.hot = (HotRec*)p;
-  p += nrec * sizeof(HotRec);
+  p += nall * sizeof(HotRec);
   v.cpa = (int32_t*)p;
   p += nrec * sizeof(int32_t);
   v.order = (int32_t*)p;
@@ -143,6 +149,7 @@ struct KArgs {
   int32_t with_contacts;
   int32_t mode;
   int32_t logcap;          // expansions logged per step per wavefront, 0 = no replay
+  int32_t arena_rounds;    // speculative rounds write their subtree to the arena
   unsigned char* arena;    // one arena_bytes(logcap) slice per workgroup
 };
 

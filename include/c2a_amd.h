@@ -63,7 +63,8 @@ typedef struct {
   double contact_scale;
   double contact_offset;
   int32_t spec_depth;       /* speculative subtree depth per round, 0 = auto */
-  int32_t groups_per_wave;  /* queries per wavefront, 0 = auto */
+  int32_t waves_per_query;  /* wavefronts cooperating on one query; this
+                               build provides 1 (0 = 1) */
   int32_t replay_log;       /* expansions logged per CA step for the next step's
                                replay (default 8192); 0 disables replay */
   int32_t pad;

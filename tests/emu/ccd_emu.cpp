@@ -8,7 +8,16 @@
 #include <vector>
 
 #include <stdint.h>
-static int64_t g_kind[2][4];   // rounds by reason, first step / later steps
+#include <stdio.h>
+#include <stdlib.h>
+static int64_t g_kind[2][4];
+#define C2A_CHECK(cond)                                                      \
+  do {                                                                       \
+    if (!(cond)) {                                                           \
+      fprintf(stderr, "ccd_emu: check failed at %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+      abort();                                                               \
+    }                                                                        \
+  } while (0)   // rounds by reason, first step / later steps
 #define C2A_EMU_COUNT(kind, L) (g_kind[(L).numCA > 1 ? 1 : 0][kind]++)
 #include "../../c2a_amd/csrc/bvh.h"
 #include "../../c2a_amd/csrc/ccd_core.h"
@@ -17,12 +26,14 @@ using namespace c2a;
 
 static int64_t g_rounds = 0;   // speculative rounds of the last batch
 extern "C" int64_t emu_rounds() { return g_rounds; }
+static int emu_arena_rounds = 1;   // the CPU harness exercises the arena-round path
+extern "C" void emu_set_arena_rounds(int on) { emu_arena_rounds = on; }
 extern "C" int64_t emu_round_kind(int later, int kind) { return g_kind[later][kind]; }
 
-template <int K>
+template <int K, int CAP>
 static int run(const KArgs& a) {
   std::vector<unsigned char> arena(a.logcap > 0 ? arena_bytes(a.logcap) : 1);
-  using W = Wave<K, 96>;
+  using W = Wave<K, CAP>;
   std::unique_ptr<W> gp(new W());
   W& g = *gp;
   memset(&g, 0, sizeof(W));
@@ -36,7 +47,7 @@ static int run(const KArgs& a) {
     leader_run(g, L, a, 0, 1);
     if (L.state == ST_DONE) break;
     g_rounds++;
-    round_lane(g, 0, 1, a);
+    round_lane(g, L, 0, 1, a);
   }
   return 0;
 }
@@ -79,12 +90,15 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
   a.with_contacts = p->with_contacts;
   a.mode = MODE_QUERY;
   a.logcap = p->replay_log;
+  a.arena_rounds = emu_arena_rounds;
+  // stack capacity as the library picks it
+  const bool small = b1.max_depth + b2.max_depth + 2 <= 48;
   switch (K) {
-    case 1: return run<1>(a);
-    case 2: return run<2>(a);
-    case 3: return run<3>(a);
-    case 4: return run<4>(a);
-    case 5: return run<5>(a);
-    default: return run<6>(a);
+    case 1: return small ? run<1, 48>(a) : run<1, 96>(a);
+    case 2: return small ? run<2, 48>(a) : run<2, 96>(a);
+    case 3: return small ? run<3, 48>(a) : run<3, 96>(a);
+    case 4: return small ? run<4, 48>(a) : run<4, 96>(a);
+    case 5: return small ? run<5, 48>(a) : run<5, 96>(a);
+    default: return small ? run<6, 48>(a) : run<6, 96>(a);
   }
 }
