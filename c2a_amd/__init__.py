@@ -48,8 +48,8 @@ class StepQuery(C.Structure):
     _fields_ = [("R1", C.c_double * 9), ("T1", C.c_double * 3), ("R2", C.c_double * 9),
                 ("T2", C.c_double * 3), ("pose_s", (C.c_double * 12) * 2),
                 ("pose_e", (C.c_double * 12) * 2), ("mesh", C.c_int32 * 2),
-                ("seed_tri", C.c_int32 * 2), ("numCA", C.c_int32), ("pad", C.c_int32),
-                ("mint_prev", C.c_double), ("UpboundTOC", C.c_double)]
+                ("seed_tri", C.c_int32 * 2), ("last_tri", C.c_int32 * 2), ("numCA", C.c_int32),
+                ("pad", C.c_int32), ("mint_prev", C.c_double), ("UpboundTOC", C.c_double)]
 
 
 class StepResult(C.Structure):

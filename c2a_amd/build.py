@@ -18,7 +18,7 @@ LIB = os.path.join(LIBDIR, "libc2a_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["ccd_kernel.hip", "capi.cpp"]
-CXX_SOURCES = ["bvh.cpp"]
+CXX_SOURCES = ["bvh.cpp", "compat.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",
           "-Wno-unused-function", "-Wno-unused-variable"]

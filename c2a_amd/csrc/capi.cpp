@@ -433,7 +433,9 @@ int c2a_ccd_step(c2a_mesh* const* meshes, int32_t nmesh, const c2a_step_query* q
       if (s >= tab[(size_t)m].ntri) return fail(C2A_ERR_BAD_ARG, "seed triangle out of range");
       d.mesh[k] = m;
       d.seed[k] = s;
-      d.last[k] = s;
+      const int l = x.last_tri[k];
+      if (l >= tab[(size_t)m].ntri) return fail(C2A_ERR_BAD_ARG, "last triangle out of range");
+      d.last[k] = l < 0 ? s : l;
       pairs[(size_t)i * 2 + k] = m;
       double R0[9], T0[3], R1[9], T1[3];
       unpack_pose(x.pose_s[k], R0, T0);

@@ -1,0 +1,88 @@
+// C2A.h -- source-compatible CCD interface of the reference (C2A/C2A.h:9-35,
+// 161-289; C2A_TimeOfContactStep from C2A.cpp:46-49, 1778-1789), implemented
+// on the MI355X engine through the C ABI in c2a_amd.h.  An existing caller
+// recompiles against these headers and links libc2a_amd.so.
+//
+// Semantics kept from the reference: C2A_Solve runs the CA query with
+// tolerance_d = tolerance_t = 1e-4 (th_ca is ignored, C2A.cpp:2384-2385),
+// seeds the first distance from dres.last_triA/B (the caller sets them, as
+// the demo does from the models' last_tri), updates each model's last_tri,
+// returns TOCFound, and for a collision writes the TOC poses and counts the
+// triangle pairs within 2 * distance + 0.001 (C2A.cpp:2411-2435).
+// Differences: no console output per call; res->R/T (the last relative
+// transform) and query_time_secs are not filled; contact pairs are counted,
+// not listed (cont_l stays empty).
+#ifndef C2A_COMPAT_C2A_H
+#define C2A_COMPAT_C2A_H
+
+#include <list>
+
+#include "C2A_Internal.h"
+#include "InterpMotion.h"
+#include "LinearMath.h"
+
+enum C2A_Result { OK, TOCFound, CollisionFound, CollisionFree, CollisionNotFound };
+
+struct ContactF {   // contact pair record of the reference (not produced here)
+  int FeatureType_A, FeatureType_B;
+  int FeatureID_A[3], FeatureID_B[3];
+  int TriangleID_A, TriangleID_B;
+  PQP_REAL P_A[3], P_B[3];
+  PQP_REAL Distance;
+};
+typedef std::list<ContactF> ContactFList;
+
+struct C2A_TimeOfContactResult {
+  int num_bv_tests;
+  int num_tri_tests;
+  double query_time_secs;
+  ContactFList cont_l;
+  PQP_REAL R[3][3];
+  PQP_REAL T[3];
+  PQP_REAL rel_err;
+  PQP_REAL abs_err;
+  PQP_REAL UpboundTOC;
+  PQP_REAL distance;
+  PQP_REAL p1[3];
+  PQP_REAL p2[3];
+  int qsize;
+  PQP_REAL mint;
+  PQP_REAL toc;          // 0 if collision free (C2A.cpp:2098, 2121)
+  bool collisionfree;
+  int numCA;
+  PQP_REAL R_toc[3][3], T_toc[3];
+  int num_contact;
+  Tri* last_triA;        // seed pair of the first distance (caller-owned)
+  Tri* last_triB;
+
+  C2A_TimeOfContactResult();
+  int NumBVTests() { return num_bv_tests; }
+  int NumTriTests() { return num_tri_tests; }
+  double QueryTimeSecs() { return query_time_secs; }
+  PQP_REAL Distance() { return distance; }
+  const PQP_REAL* P1() { return p1; }
+  const PQP_REAL* P2() { return p2; }
+};
+
+C2A_Result C2A_Solve(Transform* trans00, Transform* trans01, C2A_Model* obj1_tested,
+                     Transform* trans10, Transform* trans11, C2A_Model* obj2_tested,
+                     Transform& trans0, Transform& trans1, PQP_REAL& time_of_contact,
+                     int& number_of_iteration, int& number_of_contact, PQP_REAL th_ca,
+                     C2A_TimeOfContactResult& dres);
+
+PQP_REAL C2A_QueryTimeOfContact(CInterpMotion* objmotion1, CInterpMotion* objmotion2,
+                                C2A_TimeOfContactResult* res, C2A_Model* o1, C2A_Model* o2,
+                                PQP_REAL tolerance_d, PQP_REAL tolerance_t, int qsize = 2);
+
+// One CA step at the given poses (the defined 11-argument form, C2A.cpp:46-49).
+int C2A_TimeOfContactStep(CInterpMotion* objmotion1, CInterpMotion* objmotion2,
+                          C2A_TimeOfContactResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3],
+                          C2A_Model* o1, PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2,
+                          PQP_REAL tolerance_t, PQP_REAL tolerance_d);
+
+// Counts the triangle pairs within `threshold` at the motions' current poses.
+PQP_REAL C2A_QueryContact(CInterpMotion* objmotion1, CInterpMotion* objmotion2,
+                          C2A_TimeOfContactResult* res, C2A_Model* o1, C2A_Model* o2,
+                          double threshold);
+
+#endif

@@ -140,6 +140,8 @@ typedef struct {
   double pose_s[2][12], pose_e[2][12]; /* start/end poses defining the motions */
   int32_t mesh[2];
   int32_t seed_tri[2];
+  int32_t last_tri[2];  /* the models' last_tri before the step (kept when no
+                           leaf improves the bound); -1 = seed_tri */
   int32_t numCA;
   int32_t pad;
   double mint_prev;

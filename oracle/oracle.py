@@ -98,6 +98,10 @@ def lib():
         L.c2o_query_contact.argtypes = [C.POINTER(Model), C.POINTER(Model), dp, dp, dp, dp,
                                         C.c_double, C.POINTER(Contact), C.c_int,
                                         C.POINTER(C.c_longlong)]
+        L.c2o_toc_step.restype = None
+        L.c2o_toc_step.argtypes = [C.POINTER(Model), C.POINTER(Model), C.POINTER(Motion),
+                                   C.POINTER(Motion), dp, dp, dp, dp, C.POINTER(Model),
+                                   C.POINTER(Model), C.POINTER(Result)]
         L.c2o_solve.restype = C.c_int
         L.c2o_solve.argtypes = [dp, dp, dp, dp, C.POINTER(Model), dp, dp, dp, dp,
                                 C.POINTER(Model), dp, dp, dp, dp, C.POINTER(Result)]
@@ -275,6 +279,20 @@ def query_contact(m1, m2, R1, T1, R2, T2, threshold, cap=0):
                                 _dp(f64(R2)), _dp(f64(T2)), float(threshold), out, int(cap),
                                 C.byref(bvt))
     return n, [out[i] for i in range(min(n, cap))], bvt.value
+
+
+def toc_step(m1, m2, mo1, mo2, R1, T1, R2, T2, seeds, numCA, mint_prev, upbound):
+    """C2A_TimeOfContactStep at explicit poses: seeds = res->last_triA/B; updates
+    the models' last_tri; returns the Result."""
+    res = Result()
+    res.last_triA, res.last_triB = int(seeds[0]), int(seeds[1])
+    res.numCA = int(numCA)
+    res.mint = float(mint_prev)
+    res.UpboundTOC = float(upbound)
+    lib().c2o_toc_step(C.byref(m1.m), C.byref(m2.m), C.byref(mo1.m), C.byref(mo2.m),
+                       _dp(f64(R1)), _dp(f64(T1)), _dp(f64(R2)), _dp(f64(T2)),
+                       C.byref(m1.m), C.byref(m2.m), C.byref(res))
+    return res
 
 
 def solve(m1, m2, pose00, pose01, pose10, pose11):

@@ -1,0 +1,73 @@
+// host_check.cpp -- TEST PROGRAM: exercises the parts of the source-compatible
+// interface that need no GPU: C2A_Model building (triangle order and ids after
+// EndModel, last_tri) and CInterpMotion_Linear integration.
+//
+// input : int32 n, n*9 doubles (one model), int32 nm, nm*24 doubles (start and
+//         end pose of each motion), int32 nt, nt doubles (times)
+// output: int32 n, n*9 doubles (triangles in hierarchy order), n int32 ids,
+//         int32 last_tri index, then per motion: axis (3), angle (1) and per
+//         time the pose (12) and (w, x, y, z, t) from integrate (7)
+#include <stdio.h>
+
+#include <vector>
+
+#include "c2a/C2A.h"
+
+int main(int argc, char** argv) {
+  if (argc != 3) return 2;
+  FILE* f = fopen(argv[1], "rb");
+  if (!f) return 2;
+  int n = 0;
+  if (fread(&n, 4, 1, f) != 1) return 3;
+  std::vector<double> t((size_t)n * 9);
+  if (fread(t.data(), 8, t.size(), f) != t.size()) return 3;
+  int nm = 0;
+  if (fread(&nm, 4, 1, f) != 1) return 3;
+  std::vector<double> mp((size_t)nm * 24);
+  if (fread(mp.data(), 8, mp.size(), f) != mp.size()) return 3;
+  int nt = 0;
+  if (fread(&nt, 4, 1, f) != 1) return 3;
+  std::vector<double> ts((size_t)nt);
+  if (fread(ts.data(), 8, ts.size(), f) != ts.size()) return 3;
+  fclose(f);
+  C2A_Model m;
+  if (m.BeginModel() != PQP_OK) return 4;
+  for (int i = 0; i < n; i++) m.AddTri(&t[9 * i], &t[9 * i + 3], &t[9 * i + 6], 1000 + i);
+  if (m.EndModel() != PQP_OK) return 4;
+  if (m.EndModel() != PQP_ERR_BUILD_OUT_OF_SEQUENCE) return 5;
+  FILE* o = fopen(argv[2], "wb");
+  fwrite(&m.num_tris, 4, 1, o);
+  for (int i = 0; i < m.num_tris; i++) {
+    fwrite(m.tris[i].p1, 8, 3, o);
+    fwrite(m.tris[i].p2, 8, 3, o);
+    fwrite(m.tris[i].p3, 8, 3, o);
+  }
+  for (int i = 0; i < m.num_tris; i++) fwrite(&m.tris[i].id, 4, 1, o);
+  int lt = m.tri_index(m.last_tri);
+  fwrite(&lt, 4, 1, o);
+  for (int k = 0; k < nm; k++) {
+    const double* a = &mp[(size_t)k * 24];
+    PQP_REAL R0[3][3], T0[3], R1[3][3], T1[3];
+    for (int i = 0; i < 3; i++) {
+      T0[i] = a[9 + i];
+      T1[i] = a[21 + i];
+      for (int j = 0; j < 3; j++) {
+        R0[i][j] = a[3 * i + j];
+        R1[i][j] = a[12 + 3 * i + j];
+      }
+    }
+    CInterpMotion_Linear mo(R0, T0, R1, T1);
+    double ax[4] = {mo.m_axis.X(), mo.m_axis.Y(), mo.m_axis.Z(), mo.m_angVel};
+    fwrite(ax, 8, 4, o);
+    for (int s = 0; s < nt; s++) {
+      PQP_REAL R[3][3], T[3], q[7];
+      mo.integrate(ts[s], R, T);
+      mo.integrate(ts[s], q);
+      fwrite(R, 8, 9, o);
+      fwrite(T, 8, 3, o);
+      fwrite(q, 8, 7, o);
+    }
+  }
+  fclose(o);
+  return 0;
+}

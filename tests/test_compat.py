@@ -1,0 +1,109 @@
+"""The reference's C++ CCD interface (include/c2a/) over the engine.
+
+The host test builds a model and motions through the source-compatible
+classes (tests/compat/host_check.cpp) and compares them with the oracle.  The
+GPU test runs the reference demo's CCD loop (tests/compat/solve_demo.cpp:
+C2A_Solve per query, last_tri carried between queries as in
+CCDDemo/mainTorusknot.cpp:312-321) and compares every output with the
+oracle's C2A_Solve run in the same order.
+"""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from c2a_amd import workloads as W
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+COMPAT = os.path.join(HERE, "compat")
+
+
+def _binary(name):
+    path = os.path.join(COMPAT, name)
+    if not os.path.exists(path):
+        import __graft_entry__
+        __graft_entry__._build_compat_demo()
+    return path
+
+
+def _run(name, payload, tmp_path):
+    inp, out = tmp_path / (name + ".in"), tmp_path / (name + ".out")
+    inp.write_bytes(payload)
+    subprocess.run([_binary(name), str(inp), str(out)], check=True, timeout=300)
+    return out.read_bytes()
+
+
+def _i32(x):
+    return struct.pack("<i", int(x))
+
+
+def test_model_and_motion_match_oracle(tmp_path):
+    tris, _ = W.torus_knot(n_seg=32, n_ring=6)
+    t9 = np.ascontiguousarray(tris.reshape(-1, 9))
+    n = len(t9)
+    P = W.knot_queries(6, 404, collide_frac=0.0)
+    motions = np.concatenate([P[0], P[1]], axis=1)   # object 1: start, end
+    ts = np.array([0.0, 0.125, 0.5, 0.9, 1.0, 1.5])
+    payload = (_i32(n) + t9.tobytes() + _i32(len(motions)) + motions.tobytes() +
+               _i32(len(ts)) + ts.tobytes())
+    b = _run("host_check", payload, tmp_path)
+    off = 0
+    (nt,) = struct.unpack_from("<i", b, off); off += 4
+    assert nt == n
+    got_tris = np.frombuffer(b, np.float64, n * 9, off).reshape(n, 9); off += n * 72
+    got_ids = np.frombuffer(b, np.int32, n, off); off += 4 * n
+    (last,) = struct.unpack_from("<i", b, off); off += 4
+    m = O.OModel(t9, ids=1000 + np.arange(n))
+    ref = m.tris()
+    assert np.array_equal(got_tris, np.array(ref["p"]).reshape(n, 9))
+    assert np.array_equal(got_ids, np.array(ref["id"]))
+    assert last == 0                       # last_tri = tris after EndModel
+    for k in range(len(motions)):
+        a = motions[k]
+        om = O.OMotion(a[:9].reshape(3, 3), a[9:12], a[12:21].reshape(3, 3), a[21:24])
+        ax = np.frombuffer(b, np.float64, 4, off); off += 32
+        assert np.array_equal(ax[:3], om.axis) and ax[3] == om.angVel
+        for t in ts:
+            R, T, q7 = om.integrate(t)
+            rec = np.frombuffer(b, np.float64, 19, off); off += 19 * 8
+            assert np.array_equal(rec[:9], R.reshape(-1)), (k, t)
+            assert np.array_equal(rec[9:12], T), (k, t)
+            assert np.array_equal(rec[12:19], q7), (k, t)
+    assert off == len(b)
+
+
+@pytest.mark.gpu
+def test_solve_demo_matches_oracle(tmp_path):
+    """Configuration 0: two 8192-triangle knots, queries in order with the
+    demo's last_tri protocol, C2A_Solve's 1e-4 tolerances."""
+    tris, _ = W.torus_knot()
+    t9 = np.ascontiguousarray(tris.reshape(-1, 9))
+    cfg = W.CONFIG_KNOT_SOLVE
+    nq = 300
+    P = [p[:nq] for p in W.knot_queries(cfg["nq"], cfg["seed"], collide_frac=0.0)]
+    poses = np.stack(P, axis=1)            # nq x 4 x 12
+    payload = (_i32(len(t9)) + t9.tobytes() + _i32(len(t9)) + t9.tobytes() + _i32(nq) +
+               np.ascontiguousarray(poses).tobytes())
+    print("solve_demo: %d queries" % nq, flush=True)
+    b = _run("solve_demo", payload, tmp_path)
+    rec = np.dtype([("td", "<f8", 2), ("i5", "<i4", 5), ("tr", "<f8", 24)])
+    got = np.frombuffer(b, rec)
+    assert len(got) == nq
+    m1, m2 = O.OModel(t9), O.OModel(t9)
+    for i in range(nq):
+        p = poses[i]
+        res, (R0, T0), (R1, T1) = O.solve(m1, m2, (p[0, :9].reshape(3, 3), p[0, 9:]),
+                                          (p[1, :9].reshape(3, 3), p[1, 9:]),
+                                          (p[2, :9].reshape(3, 3), p[2, 9:]),
+                                          (p[3, :9].reshape(3, 3), p[3, 9:]))
+        g = got[i]
+        assert g["td"][0] == res.toc and g["td"][1] == res.distance, i
+        assert g["i5"][0] == res.numCA and g["i5"][2] == res.collisionfree, i
+        assert g["i5"][3] == m1.m.last_tri and g["i5"][4] == m2.m.last_tri, i
+        if not res.collisionfree:
+            assert g["i5"][1] == res.num_contact, i
+            assert np.array_equal(g["tr"][:9], R0.reshape(-1)) and np.array_equal(g["tr"][9:12], T0), i
+            assert np.array_equal(g["tr"][12:21], R1.reshape(-1)) and np.array_equal(g["tr"][21:], T1), i
