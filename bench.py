@@ -34,7 +34,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nq", type=int, default=4096)
+    ap.add_argument("--nq", type=int, default=0, help="queries (default: the workload's)")
+    ap.add_argument("--workload", choices=["knot", "uneven", "grazing"], default="knot",
+                    help="knot: the headline batch; uneven / grazing: the other configurations")
     ap.add_argument("--spec-depth", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -56,22 +58,67 @@ def copy_calibration():
     print(json.dumps({"copy_bytes": n * 8, "launches": 3}), flush=True)
 
 
-def cpu_baseline(P, tris, nq_sample):
-    """Oracle (reference-faithful FP64 restatement) on all host cores."""
+def cpu_baseline(W, nq_sample):
+    """Oracle (reference-faithful FP64 restatement) on all host cores, over the
+    first nq_sample queries of the same workload."""
     import numpy as np
 
     from oracle import oracle as O
     threads = min(64, os.cpu_count() or 1)
-    t9 = tris.reshape(-1, 9)
-    m1, m2 = O.OModel(t9), O.OModel(t9)
-    sel = [p[:nq_sample] for p in P]
-    seeds = np.zeros((nq_sample, 2), np.int32)
+    P = [p[:nq_sample] for p in W["P"]]
+    mp = W["pairs"][:nq_sample]
+    used = sorted(set(mp.reshape(-1).tolist()))
+    models = {i: O.OModel(W["tris"][i]) for i in used}
+    twins = {i: O.OModel(W["tris"][i]) for i in used}   # the second object of a self pair
     t0 = time.perf_counter()
-    O.batch_toc(m1, m2, *sel, 1e-3, 1e-4, seeds=seeds, nthreads=threads)
+    for a, b in sorted(set(map(tuple, mp))):
+        sel = np.nonzero((mp[:, 0] == a) & (mp[:, 1] == b))[0]
+        mb = models[b] if a != b else twins[b]
+        O.batch_toc(models[a], mb, *[p[sel] for p in P], W["tol_d"], W["tol_t"],
+                    seeds=np.zeros((len(sel), 2), np.int32), nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": nq_sample / dt, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": "first %d queries of the same knot-pair batch, %d threads, %.2f s wall"
+            "sample": "first %d queries of the same workload, %d threads, %.2f s wall"
                       % (nq_sample, threads, dt)}
+
+
+def load_workload(name, nq):
+    """Synthetic inputs of one configuration (host arrays)."""
+    import numpy as np
+
+    from c2a_amd import workloads as W
+    if name == "knot":
+        cfg = W.CONFIG_KNOT_BATCH
+        nq = nq or cfg["nq"]
+        tris, idx = W.torus_knot()
+        P = W.knot_queries(nq, cfg["seed"], collide_frac=cfg["collide_frac"])
+        return dict(tris=[tris.reshape(-1, 9)] * 2, vidx=[idx] * 2, P=P,
+                    pairs=np.tile(np.array([0, 1], np.int32), (nq, 1)), tol_d=cfg["tol_d"],
+                    tol_t=cfg["tol_t"], nq=nq, sample=nq,
+                    desc="knot-pair batch: %d CCD queries between two (2,3) torus-knot tubes "
+                         "(8192 triangles each), tol_d 1e-3, tol_t 1e-4, 30%% pulled to collide"
+                         % nq)
+    if name == "uneven":
+        cfg = W.CONFIG_UNEVEN
+        nq = nq or cfg["nq"]
+        tab = W.uneven_meshes()
+        sizes = [len(t) for t, _, _ in tab]
+        P00, P01, P10, P11, mp = W.uneven_queries(nq, cfg["seed"], sizes, [r for _, _, r in tab])
+        return dict(tris=[t for t, _, _ in tab], vidx=[i for _, i, _ in tab],
+                    P=(P00, P01, P10, P11), pairs=mp, tol_d=cfg["tol_d"], tol_t=cfg["tol_t"],
+                    nq=nq, sample=min(nq, 4096),
+                    desc="uneven batch: %d queries over icospheres of 80..81,920 triangles "
+                         "(value noise 0.1), a 64x32 torus and the knot; log-uniform mesh "
+                         "sizes, rigid motions spanning 0.5-4 body radii" % nq)
+    cfg = W.CONFIG_GRAZING
+    nq = nq or cfg["nq"]
+    s, idx = W.uv_sphere(cfg["n_lon"], cfg["n_lat"], cfg["noise"], 3)
+    P = W.grazing_queries(nq, cfg["seed"])
+    return dict(tris=[s.reshape(-1, 9)] * 2, vidx=[idx] * 2, P=P,
+                pairs=np.tile(np.array([0, 1], np.int32), (nq, 1)), tol_d=cfg["tol_d"],
+                tol_t=cfg["tol_t"], nq=nq, sample=nq,
+                desc="grazing contacts: %d near-tangential passes between two noisy 512x256 UV "
+                     "spheres (262,144 triangles each), closest approach within 5e-3 radii" % nq)
 
 
 def main():
@@ -87,20 +134,20 @@ def main():
     import torch
 
     import c2a_amd as G
-    from c2a_amd import workloads as W
 
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
-    cfg = W.CONFIG_KNOT_BATCH
-    nq = args.nq
-    tris, idx = W.torus_knot()
-    t9 = tris.reshape(-1, 9)
-    meshes = [G.Mesh(t9, vidx=idx), G.Mesh(t9, vidx=idx)]
-    P = W.knot_queries(nq, cfg["seed"], collide_frac=cfg["collide_frac"])
-    params = G.default_params(tol_d=cfg["tol_d"], tol_t=cfg["tol_t"], with_contacts=0,
+    wl = load_workload(args.workload, args.nq)
+    nq = wl["nq"]
+    if args.workload == "knot":
+        t9, idx = wl["tris"][0], wl["vidx"][0]
+        meshes = [G.Mesh(t9, vidx=idx), G.Mesh(t9, vidx=idx)]
+    else:
+        meshes = [G.Mesh(t, vidx=i) for t, i in zip(wl["tris"], wl["vidx"])]
+    params = G.default_params(tol_d=wl["tol_d"], tol_t=wl["tol_t"], with_contacts=0,
                               spec_depth=args.spec_depth)
-    batch = G.Batch(meshes, G.make_queries(*P), params)
+    batch = G.Batch(meshes, G.make_queries(*wl["P"], mesh_pairs=wl["pairs"]), params)
 
     for _ in range(args.warmup):
         batch.run()
@@ -128,7 +175,7 @@ def main():
     achieved = algo_bytes / (kms / 1000.0) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "ccd_kernel_knot4096_traffic.json")
-    if os.path.exists(tpath):
+    if args.workload == "knot" and nq == 4096 and os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
         except Exception:
@@ -147,8 +194,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "knot-pair batch: 4096 CCD queries between two (2,3) torus-knot tubes "
-                        "(8192 triangles each), tol_d 1e-3, tol_t 1e-4, 30% pulled to collide",
+            "workload": wl["desc"],
             "order": "plain (reference DFS order, bit-exact)",
             "nq": nq,
             "spec_depth": args.spec_depth or 5,
@@ -169,7 +215,7 @@ def main():
         },
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(P, tris, nq)
+        out["cpu_baseline"] = cpu_baseline(wl, wl["sample"])
     print(json.dumps(out), flush=True)
 
 

@@ -250,3 +250,99 @@ CONFIG_KNOT_SOLVE = dict(name="knot-pair sequential C2A_Solve parity", nq=1000, 
                          tol_d=1e-3, tol_t=1e-4, collide_frac=0.0)
 CONFIG_KNOT_BATCH = dict(name="knot-pair batched CCD", nq=4096, seed=20261,
                          tol_d=1e-3, tol_t=1e-4, collide_frac=0.3)
+
+
+# ---------------------------------------------------------------------------
+# uneven batch: many meshes of very different sizes
+# ---------------------------------------------------------------------------
+def uneven_meshes(levels=(1, 2, 3, 4, 5, 6), seed=7):
+    """Mesh table of the uneven batch: noise-displaced icospheres (amplitude
+    0.1) at the given subdivision levels (80 * 4^(L-1) triangles: 80 ... 81,920),
+    a 64x32 torus and the (2,3) knot tube.  Returns a list of (tris9, vidx,
+    bounding radius about the origin)."""
+    out = []
+    for lv in levels:
+        t, idx = icosphere(lv, 0.1, seed + lv)
+        out.append((t.reshape(-1, 9), idx))
+    t, idx = torus(64, 32)
+    out.append((t.reshape(-1, 9), idx))
+    t, idx = torus_knot()
+    out.append((t.reshape(-1, 9), idx))
+    return [(t, idx, float(np.sqrt((t.reshape(-1, 3) ** 2).sum(1)).max())) for t, idx in out]
+
+
+def _rotation_about(axis, angle):
+    axis = axis / np.linalg.norm(axis, axis=1, keepdims=True)
+    h = 0.5 * angle
+    q = np.concatenate([axis * np.sin(h)[:, None], np.cos(h)[:, None]], axis=1)
+    return quat_to_matrix(q)
+
+
+def uneven_queries(nq, seed, sizes, radii):
+    """Queries over a mesh table: each object's mesh is drawn log-uniformly in
+    triangle count (nearest table entry in log size), and each object moves
+    rigidly: random start rotation, end rotation = start turned about a random
+    axis by U[0, pi], translation span U[0.5, 4] body radii along a random
+    direction.  Object 2 starts U[1, 3] combined radii from object 1 and its
+    path is aimed at object 1's end position with probability 0.5.  Returns
+    (P00, P01, P10, P11, mesh_pairs)."""
+    rng = Rng(seed, 3)
+    sizes = np.asarray(sizes, dtype=np.float64)
+    radii = np.asarray(radii, dtype=np.float64)
+    ls = np.log(sizes)
+    pick = rng.uniform(2 * nq, ls.min(), ls.max()).reshape(nq, 2)
+    mp = np.abs(pick[:, :, None] - ls[None, None, :]).argmin(axis=2).astype(np.int32)
+    r1, r2 = radii[mp[:, 0]], radii[mp[:, 1]]
+
+    def unit(n):
+        v = rng.uniform(3 * n, -1.0, 1.0).reshape(n, 3)
+        return v / np.maximum(np.linalg.norm(v, axis=1, keepdims=True), 1e-12)
+
+    def motion(r, start):
+        R0 = quat_to_matrix(random_quats(rng, nq))
+        R1 = np.einsum("nij,njk->nik", _rotation_about(unit(nq), rng.uniform(nq, 0.0, np.pi)), R0)
+        T1 = start + unit(nq) * (rng.uniform(nq, 0.5, 4.0) * r)[:, None]
+        return R0, start, R1, T1
+
+    R00, T00, R01, T01 = motion(r1, np.zeros((nq, 3)))
+    start2 = unit(nq) * (rng.uniform(nq, 1.0, 3.0) * (r1 + r2))[:, None]
+    R10, T10, R11, T11 = motion(r2, start2)
+    aim = rng.uniform(nq) < 0.5
+    T11 = np.where(aim[:, None], T01 + unit(nq) * (0.5 * r1)[:, None], T11)
+    return (pack_poses(R00, T00), pack_poses(R01, T01), pack_poses(R10, T10),
+            pack_poses(R11, T11), mp)
+
+
+# ---------------------------------------------------------------------------
+# large grazing contacts
+# ---------------------------------------------------------------------------
+def grazing_queries(nq, seed, radius=1.0, max_sep=5e-3):
+    """Two unit-size bodies, object 1 fixed at the origin turning slowly,
+    object 2 passing it on a straight line whose closest approach between the
+    centers is 2*radius*(1 + s), s ~ U[0, max_sep] (near-tangential motion),
+    while turning about a random axis.  Returns (P00, P01, P10, P11)."""
+    rng = Rng(seed, 5)
+
+    def unit(n):
+        v = rng.uniform(3 * n, -1.0, 1.0).reshape(n, 3)
+        return v / np.maximum(np.linalg.norm(v, axis=1, keepdims=True), 1e-12)
+
+    R00 = quat_to_matrix(random_quats(rng, nq))
+    R01 = np.einsum("nij,njk->nik", _rotation_about(unit(nq), rng.uniform(nq, 0.0, 0.2)), R00)
+    T0 = np.zeros((nq, 3))
+    n = unit(nq)                                  # direction of closest approach
+    d = unit(nq)
+    d = d - (d * n).sum(1, keepdims=True) * n     # travel direction, orthogonal to n
+    d = d / np.linalg.norm(d, axis=1, keepdims=True)
+    sep = rng.uniform(nq, 0.0, max_sep)
+    c = n * (2 * radius * (1 + sep))[:, None]
+    half = rng.uniform(nq, 1.0, 2.0)[:, None] * radius
+    R10 = quat_to_matrix(random_quats(rng, nq))
+    R11 = np.einsum("nij,njk->nik", _rotation_about(unit(nq), rng.uniform(nq, 0.0, 1.0)), R10)
+    return (pack_poses(R00, T0), pack_poses(R01, T0), pack_poses(R10, c - half * d),
+            pack_poses(R11, c + half * d))
+
+
+CONFIG_UNEVEN = dict(name="uneven batched CCD", nq=16384, seed=20262, tol_d=1e-3, tol_t=1e-4)
+CONFIG_GRAZING = dict(name="large grazing-contact stress", nq=1024, seed=20264, tol_d=1e-3,
+                      tol_t=1e-4, n_lon=512, n_lat=256, noise=0.1)
