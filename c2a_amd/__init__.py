@@ -73,7 +73,9 @@ EXPORTS = ["c2a_params_default", "c2a_mesh_create", "c2a_mesh_destroy", "c2a_mes
            "c2a_mesh_tri_ids", "c2a_mesh_export", "c2a_ccd_batch", "c2a_batch_create",
            "c2a_batch_run", "c2a_batch_fetch", "c2a_batch_destroy", "c2a_batch_last_ms",
            "c2a_ccd_step", "c2a_contact_count", "c2a_motion_prepare", "c2a_motion_pose",
-           "c2a_status_string", "c2a_last_error"]
+           "c2a_status_string", "c2a_last_error", "c2a_scorer_create", "c2a_scorer_destroy",
+           "c2a_scorer_set_params", "c2a_scorer_get_params", "c2a_scorer_infer",
+           "c2a_scorer_grad", "c2a_scorer_train"]
 
 
 def lib():
@@ -257,6 +259,58 @@ def contact_count(meshes, poses, mesh_pairs, thresholds):
     _check(lib().c2a_contact_count(_mesh_table(meshes), len(meshes), _dp(poses), _ip(mp),
                                    _dp(th), len(th), _ip(out), None))
     return out
+
+
+class Scorer:
+    """The learned front-ordering pair scorer (GPU parameters)."""
+
+    NPARAM = 6337
+
+    def __init__(self, seed=0):
+        h = C.c_void_p()
+        _check(lib().c2a_scorer_create(C.c_uint64(seed), C.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().c2a_scorer_destroy(self.handle)
+            self.handle = None
+
+    def params(self):
+        p = np.zeros(self.NPARAM, dtype=np.float32)
+        _check(lib().c2a_scorer_get_params(self.handle, p.ctypes.data_as(C.c_void_p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().c2a_scorer_set_params(self.handle, p.ctypes.data_as(C.c_void_p)))
+
+    def infer(self, feats):
+        x = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, 32)
+        out = np.zeros(len(x), dtype=np.float32)
+        _check(lib().c2a_scorer_infer(self.handle, x.ctypes.data_as(C.c_void_p), C.c_int64(len(x)),
+                                      out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def grad(self, feats, labels):
+        x = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, 32)
+        y = np.ascontiguousarray(labels, dtype=np.float32)
+        g = np.zeros(self.NPARAM, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().c2a_scorer_grad(self.handle, x.ctypes.data_as(C.c_void_p),
+                                     y.ctypes.data_as(C.c_void_p), C.c_int64(len(y)),
+                                     g.ctypes.data_as(C.c_void_p), C.byref(loss)))
+        return loss.value, g
+
+    def train(self, feats, labels, epochs=10, batch=1024, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, 32)
+        y = np.ascontiguousarray(labels, dtype=np.float32)
+        losses = np.zeros(epochs, dtype=np.float32)
+        _check(lib().c2a_scorer_train(self.handle, x.ctypes.data_as(C.c_void_p),
+                                      y.ctypes.data_as(C.c_void_p), C.c_int64(len(y)),
+                                      int(epochs), int(batch), C.c_float(lr), C.c_uint64(seed),
+                                      losses.ctypes.data_as(C.c_void_p)))
+        return losses
 
 
 def motion_prepare(pose_start, pose_end):

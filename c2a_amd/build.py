@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libc2a_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["ccd_kernel.hip", "capi.cpp"]
+HIP_SOURCES = ["ccd_kernel.hip", "capi.cpp", "scorer.hip"]
 CXX_SOURCES = ["bvh.cpp", "compat.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",

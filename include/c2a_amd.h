@@ -172,6 +172,28 @@ int c2a_motion_prepare(const double *pose_start, const double *pose_end, double 
  * pose12 = R[9] then T[3]; quat4 (optional) = (x,y,z,w). */
 int c2a_motion_pose(const double *motion24, double t, double *pose12, double *quat4);
 
+/* Learned front ordering (no counterpart in the reference; its north-star
+ * extension, the pair scorer used to order the traversal).  A scorer is a
+ * 32 -> 64 -> 64 -> 1 ReLU network with BF16 weights on the matrix cores and
+ * FP32 accumulation and master parameters (6337 FP32 values: W1[32][64],
+ * b1[64], W2[64][64], b2[64], w3[64], b3).  The feature list is frozen in
+ * c2a_amd/csrc/scorer.h; the target is log(closest triangle distance in the
+ * pair's subtree + 1e-4). */
+typedef struct c2a_scorer c2a_scorer;
+int c2a_scorer_create(uint64_t seed, c2a_scorer **out);   /* He-uniform weights */
+int c2a_scorer_destroy(c2a_scorer *s);
+int c2a_scorer_set_params(c2a_scorer *s, const float *params6337);
+int c2a_scorer_get_params(const c2a_scorer *s, float *params6337);
+/* Scores of n feature rows (n x 32 FP32). */
+int c2a_scorer_infer(const c2a_scorer *s, const float *feats, int64_t n, float *out);
+/* Gradient (6337) and value of the mean squared error over n rows. */
+int c2a_scorer_grad(const c2a_scorer *s, const float *feats, const float *labels, int64_t n,
+                    float *grad_out, float *loss_out);
+/* Mini-batch Adam for `epochs` passes in a seeded shuffled order; loss_out
+ * (optional) receives the mean training loss of each epoch. */
+int c2a_scorer_train(c2a_scorer *s, const float *feats, const float *labels, int64_t n,
+                     int32_t epochs, int32_t batch, float lr, uint64_t seed, float *loss_out);
+
 const char *c2a_status_string(int status);
 /* Thread-local text of the last error, "" if none. */
 const char *c2a_last_error(void);
