@@ -16,6 +16,8 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "c2a_amd.h")
 
 OK, ERR_BAD_ARG, ERR_OOM, ERR_HIP, ERR_NO_DEVICE, ERR_UNSUPPORTED, ERR_EMPTY_MODEL = range(7)
 FLAG_ITER_CAP, FLAG_TRANSLATION_ONLY, FLAG_STACK_OVERFLOW = 1, 2, 4
+ORDER_PLAIN, ORDER_LEARNED = 0, 1
+NF = 32
 
 
 class C2AError(RuntimeError):
@@ -33,7 +35,8 @@ class Params(C.Structure):
     _fields_ = [("tol_d", C.c_double), ("tol_t", C.c_double), ("max_iter", C.c_int32),
                 ("with_contacts", C.c_int32), ("contact_scale", C.c_double),
                 ("contact_offset", C.c_double), ("spec_depth", C.c_int32),
-                ("waves_per_query", C.c_int32), ("replay_log", C.c_int32), ("pad", C.c_int32)]
+                ("waves_per_query", C.c_int32), ("replay_log", C.c_int32), ("order", C.c_int32),
+                ("scorer", C.c_void_p), ("sample_rate", C.c_double), ("sample_cap", C.c_int64)]
 
 
 class Result(C.Structure):
@@ -75,7 +78,8 @@ EXPORTS = ["c2a_params_default", "c2a_mesh_create", "c2a_mesh_destroy", "c2a_mes
            "c2a_ccd_step", "c2a_contact_count", "c2a_motion_prepare", "c2a_motion_pose",
            "c2a_status_string", "c2a_last_error", "c2a_scorer_create", "c2a_scorer_destroy",
            "c2a_scorer_set_params", "c2a_scorer_get_params", "c2a_scorer_infer",
-           "c2a_scorer_grad", "c2a_scorer_train"]
+           "c2a_scorer_grad", "c2a_scorer_train", "c2a_scorer_save", "c2a_scorer_load",
+           "c2a_batch_samples"]
 
 
 def lib():
@@ -104,6 +108,9 @@ def lib():
     L.c2a_batch_fetch.argtypes = [vp, vp, vp]
     L.c2a_batch_destroy.argtypes = [vp]
     L.c2a_batch_last_ms.argtypes = [vp, C.POINTER(C.c_float)]
+    L.c2a_batch_samples.argtypes = [vp, vp, vp, vp, i64, C.POINTER(i64), C.POINTER(i64)]
+    L.c2a_scorer_save.argtypes = [vp, C.c_char_p]
+    L.c2a_scorer_load.argtypes = [C.c_char_p, C.POINTER(vp)]
     L.c2a_ccd_step.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(StepQuery), i64,
                                C.POINTER(StepResult), vp]
     L.c2a_contact_count.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, dp, i64, ip, vp]
@@ -137,9 +144,12 @@ def _ip(a):
 
 
 def default_params(**kw):
+    """c2a_params_default plus overrides; scorer= takes a Scorer (learned order)."""
     p = Params()
     lib().c2a_params_default(C.byref(p))
     for k, v in kw.items():
+        if k == "scorer":
+            v = None if v is None else v.handle
         setattr(p, k, v)
     return p
 
@@ -213,8 +223,9 @@ def ccd_batch(meshes, queries, params=None, stream=None):
 class Batch:
     """Device-resident batch: upload once, run many times (benchmarking)."""
 
-    def __init__(self, meshes, queries, params=None):
+    def __init__(self, meshes, queries, params=None, scorer=None):
         self._meshes = meshes
+        self._scorer = scorer   # keeps a learned-order scorer alive with the batch
         self.nq = len(queries)
         p = params if params is not None else default_params()
         h = C.c_void_p()
@@ -234,6 +245,18 @@ class Batch:
         ms = C.c_float()
         _check(lib().c2a_batch_last_ms(self.handle, C.byref(ms)))
         return ms.value
+
+    def samples(self, cap):
+        """Training samples of the last run: (features (n, 32) float32, labels (n,),
+        keys (n,) int64, selected count)."""
+        x = np.zeros((cap, NF), dtype=np.float32)
+        y = np.zeros(cap, dtype=np.float32)
+        k = np.zeros(cap, dtype=np.int64)
+        n, w = C.c_int64(), C.c_int64()
+        _check(lib().c2a_batch_samples(self.handle, x.ctypes.data_as(C.c_void_p),
+                                       y.ctypes.data_as(C.c_void_p), k.ctypes.data_as(C.c_void_p),
+                                       C.c_int64(cap), C.byref(n), C.byref(w)))
+        return x[:w.value], y[:w.value], k[:w.value], n.value
 
     def __del__(self):
         if getattr(self, "handle", None):
@@ -266,10 +289,16 @@ class Scorer:
 
     NPARAM = 6337
 
-    def __init__(self, seed=0):
+    def __init__(self, seed=0, path=None):
         h = C.c_void_p()
-        _check(lib().c2a_scorer_create(C.c_uint64(seed), C.byref(h)))
+        if path is None:
+            _check(lib().c2a_scorer_create(C.c_uint64(seed), C.byref(h)))
+        else:
+            _check(lib().c2a_scorer_load(os.fsencode(path), C.byref(h)))
         self.handle = h
+
+    def save(self, path):
+        _check(lib().c2a_scorer_save(self.handle, os.fsencode(path)))
 
     def __del__(self):
         if getattr(self, "handle", None):

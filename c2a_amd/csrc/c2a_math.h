@@ -13,7 +13,9 @@
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
-#define C2A_HD __host__ __device__ inline
+// Forced inlining: the CCD kernel is one persistent loop, and an outlined
+// call in it costs a scratch-resident frame and register save/restore.
+#define C2A_HD __host__ __device__ inline __attribute__((always_inline))
 #else
 #define C2A_HD inline
 #endif

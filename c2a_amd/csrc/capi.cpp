@@ -17,6 +17,11 @@
 #include "bvh.h"
 #include "c2a_math.h"
 #include "ccd_kernel.h"
+#include "scorer.h"
+
+namespace c2a {
+ScorerDev scorer_view(const c2a_scorer* s);   // scorer.hip
+}
 
 using namespace c2a;
 
@@ -81,11 +86,13 @@ struct c2a_mesh {
 
 struct c2a_batch {
   std::vector<MeshDev> mtab;
-  DevBuf d_mesh, d_q, d_out, d_counter, d_arena;
+  DevBuf d_mesh, d_q, d_out, d_counter, d_arena, d_samples, d_keys, d_scount;
   int64_t nq = 0;
   KArgs args{};
   int K = 5, CAP = 48, NW = 4, blocks = 1;
   int logcap = 0;
+  const c2a_scorer* scorer = nullptr;   // learned order
+  int64_t sample_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
   ~c2a_batch() {
@@ -138,20 +145,26 @@ int pick_cap(const std::vector<MeshDev>& tab, const int32_t* pairs, int64_t n, i
 int pick_shape(const c2a_params* p, int& K, int& NW) {
   K = (p && p->spec_depth > 0) ? p->spec_depth : 5;
   NW = (p && p->waves_per_query > 0) ? p->waves_per_query : 1;
+  const bool learned = p && p->order == C2A_ORDER_LEARNED;
+  if (p && p->order != C2A_ORDER_PLAIN && !learned)
+    return fail(C2A_ERR_BAD_ARG, "order must be C2A_ORDER_PLAIN or C2A_ORDER_LEARNED");
+  if (learned && !p->scorer) return fail(C2A_ERR_BAD_ARG, "learned order needs a scorer");
   if (K < 1 || K > 6) return fail(C2A_ERR_BAD_ARG, "spec_depth must be 1..6");
-  if (!kernel_shape_ok(K, 48, NW)) return fail(C2A_ERR_BAD_ARG, "waves_per_query must be 0 or 1");
+  if (!kernel_shape_ok(K, 48, NW, false)) return fail(C2A_ERR_BAD_ARG, "waves_per_query must be 0 or 1");
+  if (learned && !kernel_shape_ok(K, 48, NW, true))
+    return fail(C2A_ERR_BAD_ARG, "learned order runs at spec_depth 0 or 5");
   return C2A_OK;
 }
 
 // Resident workgroups: the kernel is persistent, so launching more than fit
 // at once only adds workgroups that find the queue empty.
-int grid_blocks(int K, int CAP, int NW, int64_t nq) {
+int grid_blocks(int K, int CAP, int NW, int64_t nq, bool learned = false) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   hipDeviceProp_t prop;
   int cus = 256;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-  int per_cu = kernel_blocks_per_cu(K, CAP, NW);
+  int per_cu = kernel_blocks_per_cu(K, CAP, NW, learned);
   if (per_cu < 1) per_cu = 1;
   const int64_t cap = (int64_t)cus * per_cu;
   return (int)std::max<int64_t>(1, std::min<int64_t>(nq, cap));
@@ -201,6 +214,15 @@ int fill_args(c2a_batch* b, const c2a_params* p) {
   a.mode = MODE_QUERY;
   a.logcap = b->logcap;
   a.arena = (unsigned char*)b->d_arena.p;
+  a.feats = (b->scorer || b->sample_cap > 0) ? 1 : 0;
+  if (b->sample_cap > 0) {
+    const double r = p->sample_rate < 0 ? 0.0 : (p->sample_rate > 1 ? 1.0 : p->sample_rate);
+    a.sample_thr = (uint64_t)(r * 4294967296.0);
+    a.samples = (float*)b->d_samples.p;
+    a.sample_keys = (int64_t*)b->d_keys.p;
+    a.sample_cap = b->sample_cap;
+    a.sample_count = (unsigned long long*)b->d_scount.p;
+  }
   return C2A_OK;
 }
 
@@ -231,7 +253,10 @@ void c2a_params_default(c2a_params* p) {
   p->spec_depth = 0;
   p->waves_per_query = 0;
   p->replay_log = 8192;
-  p->pad = 0;
+  p->order = C2A_ORDER_PLAIN;
+  p->scorer = nullptr;
+  p->sample_rate = 0.0;
+  p->sample_cap = 0;
 }
 
 int c2a_mesh_create(const double* tri_xyz9, const int32_t* tri_ids, const int32_t* vert_idx3,
@@ -330,7 +355,6 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     delete b;
     return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
   }
-  b->blocks = grid_blocks(b->K, b->CAP, b->NW, nq);
   {
     c2a_params def;
     c2a_params_default(&def);
@@ -339,10 +363,22 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
       delete b;
       return fail(C2A_ERR_BAD_ARG, "replay_log must be 0..2^24");
     }
+    if (pp->sample_cap < 0 || (pp->sample_cap > 0 && pp->replay_log == 0)) {
+      delete b;
+      return fail(C2A_ERR_BAD_ARG, "sample_cap must be >= 0 and needs replay_log > 0");
+    }
     b->logcap = pp->replay_log;
+    b->scorer = pp->order == C2A_ORDER_LEARNED ? pp->scorer : nullptr;
+    b->sample_cap = pp->sample_cap;
   }
+  b->blocks = grid_blocks(b->K, b->CAP, b->NW, nq, b->scorer != nullptr);
+  const bool feats = b->scorer || b->sample_cap > 0;
   if ((b->logcap > 0 && nq > 0 &&
-       (rc = b->d_arena.alloc((size_t)b->blocks * arena_bytes(b->logcap)))) ||
+       (rc = b->d_arena.alloc((size_t)b->blocks * arena_bytes(b->logcap, feats)))) ||
+      (b->sample_cap > 0 &&
+       ((rc = b->d_samples.alloc((size_t)b->sample_cap * (NF + 1) * sizeof(float))) ||
+        (rc = b->d_keys.alloc((size_t)b->sample_cap * sizeof(int64_t))) ||
+        (rc = b->d_scount.alloc(sizeof(int64_t))))) ||
       (rc = b->d_mesh.alloc(b->mtab.size() * sizeof(MeshDev))) ||
       (rc = b->d_q.alloc((size_t)std::max<int64_t>(nq, 1) * sizeof(QueryDev))) ||
       (rc = b->d_out.alloc((size_t)std::max<int64_t>(nq, 1) * sizeof(c2a_result))) ||
@@ -369,6 +405,11 @@ int c2a_batch_run(c2a_batch* b, void* stream) {
   if (!b) return fail(C2A_ERR_BAD_ARG, "null batch");
   if (b->nq == 0) return C2A_OK;
   hipStream_t s = (hipStream_t)stream;
+  if (b->scorer) b->args.scorer = scorer_view(b->scorer);   // current parameters
+  if (b->sample_cap > 0) {
+    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, (int64_t*)b->d_scount.p);
+    HIP_TRY(hipGetLastError());
+  }
   HIP_TRY(hipEventRecord(b->ev0, s));
   int rc = run_kernel(b->args, b->K, b->CAP, b->NW, b->blocks, s);
   if (rc) return rc;
@@ -392,6 +433,43 @@ int c2a_batch_last_ms(c2a_batch* b, float* ms) {
   if (!b->timed) return fail(C2A_ERR_BAD_ARG, "batch has not run");
   HIP_TRY(hipEventSynchronize(b->ev1));
   HIP_TRY(hipEventElapsedTime(ms, b->ev0, b->ev1));
+  return C2A_OK;
+}
+
+int c2a_batch_samples(c2a_batch* b, float* feats, float* labels, int64_t* keys, int64_t cap,
+                      int64_t* n_out, int64_t* n_written) {
+  if (!b || cap < 0 || (cap > 0 && (!feats || !labels || !keys)))
+    return fail(C2A_ERR_BAD_ARG, "bad sample buffers");
+  if (b->sample_cap <= 0) return fail(C2A_ERR_BAD_ARG, "the batch logs no samples");
+  unsigned long long cnt = 0;
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(&cnt, b->d_scount.p, sizeof(cnt), hipMemcpyDeviceToHost));
+  const int64_t have = std::min<int64_t>((int64_t)cnt, b->sample_cap);
+  std::vector<float> rows;
+  std::vector<int64_t> k;
+  try {
+    rows.resize((size_t)have * (NF + 1));
+    k.resize((size_t)have);
+  } catch (const std::bad_alloc&) {
+    return fail(C2A_ERR_OOM, "host allocation failed");
+  }
+  if (have > 0) {
+    HIP_TRY(hipMemcpy(rows.data(), b->d_samples.p, rows.size() * sizeof(float),
+                      hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(k.data(), b->d_keys.p, k.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+  }
+  std::vector<int64_t> idx((size_t)have);
+  for (int64_t i = 0; i < have; i++) idx[(size_t)i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return k[(size_t)x] < k[(size_t)y]; });
+  const int64_t w = std::min(have, cap);
+  for (int64_t i = 0; i < w; i++) {
+    const int64_t j = idx[(size_t)i];
+    memcpy(feats + i * NF, &rows[(size_t)j * (NF + 1)], NF * sizeof(float));
+    labels[i] = rows[(size_t)j * (NF + 1) + NF];
+    keys[i] = k[(size_t)j];
+  }
+  if (n_out) *n_out = (int64_t)cnt;
+  if (n_written) *n_written = w;
   return C2A_OK;
 }
 

@@ -195,6 +195,8 @@ struct Slot {            // speculative pair values, heap-indexed below X
   double dTri, mleaf;    // leaf pairs: triangle distance and leaf admissible time
   double p[3], q[3];     // leaf pairs: closest points in model-1 frame
   int32_t t1, t2;
+  float score;           // learned order: the scorer's output for the pair
+  int32_t spad;
 };
 
 template <int K>
@@ -207,6 +209,7 @@ template <int K, int CAP>
 struct Wave {
   Entry stack[CAP];
   Slot slot[Shape<K>::NSLOT];
+  uint16_t fx[64][NF];       // learned order: BF16 features of the round's slots
   Entry X;                   // root of the pending round
   double Rrel[9], Trel[3];   // model 2 in model 1's frame (leaf tests)
   double r1[9], tt1[3];      // current pose of object 1 (world directions)
@@ -229,7 +232,8 @@ struct Wave {
 enum : int {
   ST_FETCH = 0, ST_STEP = 1, ST_WALK = 2, ST_CONTACT_WALK = 3, ST_DONE = 4,
   ST_SCAN = 5,   // lane-parallel walk over the replayed subtree
-  ST_SUB = 6     // sequential walk below a pair the scan found unexpanded
+  ST_SUB = 6,    // sequential walk below a pair the scan found unexpanded
+  ST_LAYOUT = 7  // replayed records scored: lay them out and scan
 };
 
 // The CA loop and step state of one query; every lane holds the same values.
@@ -256,6 +260,7 @@ struct Leader {
   int wbase;                     // first record staged in the LDS window
   int spos, nev, logfull;        // scan position, events, log capacity reached
   int napp;                      // next free record pair of the arena
+  int need_score;                // replayed records wait for the scorer
   unsigned char* ab;             // this wavefront's replay arena
 };
 
@@ -350,7 +355,7 @@ C2A_HD bool descend(node_ptr N1, node_ptr N2, int& b1, int& b2, int bit, double*
 // (C2A.cpp:1237-1276); returns the leaf flags of the two nodes.
 C2A_HD int pair_values(node_ptr N1, node_ptr N2, int b1, int b2, const double* R,
                        const double* T, const double* r1, const Motion& mo1, const Motion& mo2,
-                       bool contact, double& d_out, double& mt_out) {
+                       bool contact, double& d_out, double& mt_out, double* mb_out = nullptr) {
   node_ptr A1 = N1 + b1;
   node_ptr A2 = N2 + b2;
   double S[3], la[2] = {A1->l[0], A1->l[1]}, lb[2] = {A2->l[0], A2->l[1]};
@@ -369,6 +374,10 @@ C2A_HD int pair_values(node_ptr N1, node_ptr N2, int b1, int b2, const double* R
       double mb2 = motion_bound(mo2, A2->angR, Sn);
       mt = d / (mb1 + mb2);
       if (mt <= 0) mt = 0.0;
+      if (mb_out) {
+        mb_out[0] = mb1;
+        mb_out[1] = mb2;
+      }
     }
   }
   d_out = d;
@@ -376,11 +385,27 @@ C2A_HD int pair_values(node_ptr N1, node_ptr N2, int b1, int b2, const double* R
   return node_leaf(A1) | (node_leaf(A2) << 1);
 }
 
+// Scorer features of pair (b1, b2) at transform (R, T), as BF16.  mb: the
+// two motion bounds pair_values used (zero when the pair touches).
+C2A_HD void pair_feature_row(uint16_t* out, node_ptr N1, node_ptr N2, int b1, int b2,
+                             const double* R, const double* T, double d, double mt,
+                             const double* mb, const Motion& mo1, const Motion& mo2, double up,
+                             int fl) {
+  node_ptr A1 = N1 + b1;
+  node_ptr A2 = N2 + b2;
+  const double l1[2] = {A1->l[0], A1->l[1]}, l2[2] = {A2->l[0], A2->l[1]};
+  float f[NF];
+  pair_features(f, l1, l2, A1->r, A2->r, R, T, d, mt, mb[0], mb[1], A1->depth, A2->depth,
+                A1->angR, A2->angR, mo1, mo2, up, fl & 3);
+  for (int k = 0; k < NF; k++) out[k] = f32_to_bf16(f[k]);
+}
+
 // Heap slot h of the depth-K subtree below X: the pair reached by the path
-// bits of h+1, its transform chained from X, and its values.
+// bits of h+1, its transform chained from X, and its values.  fx (optional):
+// where the pair's scorer features go (zeros for a slot below a leaf pair).
 template <int K, int CAP>
 C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
-                              bool contact, Slot& s) {
+                              bool contact, Slot& s, uint16_t* fx = nullptr, double up = 0.0) {
   node_ptr N1 = nodes_of(M1);
   node_ptr N2 = nodes_of(M2);
   const int lvl = 31 - clz32((unsigned)(h + 1));
@@ -392,12 +417,15 @@ C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const M
     const int bit = ((h + 1) >> (lvl - 1 - lv)) & 1;
     if (!descend(N1, N2, b1, b2, bit, R, T)) {
       s.valid = 0;
+      if (fx)
+        for (int k = 0; k < NF; k++) fx[k] = 0;
       return;
     }
   }
-  double d, mt;
+  double d, mt, mb[2] = {0.0, 0.0};
   C2A_CHECK(b1 >= 0 && b2 >= 0 && b1 < 2 * M1.ntri - 1 && b2 < 2 * M2.ntri - 1);
-  const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], contact, d, mt);
+  const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], contact, d, mt, mb);
+  if (fx) pair_feature_row(fx, N1, N2, b1, b2, R, T, d, mt, mb, g.mo[0], g.mo[1], up, fl);
   for (int i = 0; i < 9; i++) s.R[i] = R[i];
   for (int i = 0; i < 3; i++) s.T[i] = T[i];
   s.d = d;
@@ -412,8 +440,8 @@ C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const M
 
 template <int K, int CAP>
 C2A_HD void compute_slot(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
-                         bool contact) {
-  compute_slot_into(g, h, M1, M2, contact, g.slot[h]);
+                         bool contact, uint16_t* fx = nullptr, double up = 0.0) {
+  compute_slot_into(g, h, M1, M2, contact, g.slot[h], fx, up);
 }
 
 // Heap slot h of the round written as arena record 2 * rpb + h, so that the
@@ -444,8 +472,16 @@ C2A_HD void compute_slot_arena(Wave<K, CAP>& g, int h, const MeshDev& M1, const 
   hr.cp = (!leaf && lvl < K) ? rpb + h : -1;
   hr.t1 = leaf ? s.t1 : -1;
   hr.t2 = leaf ? s.t2 : -1;
-  hr.pad[0] = hr.pad[1] = 0;
+  hr.score = 0.0f;
+  hr.pad = 0;
   gp(av.hot)[rec] = hr;
+}
+
+// Whether the pending round's slots are ordered by the scorer: CA-step
+// rounds below an internal pair (the scorer covers up to 64 slots).
+template <int K, int CAP>
+C2A_HD bool round_scored(const Wave<K, CAP>& g, const KArgs& a) {
+  return Shape<K>::NSLOT <= 64 && a.scorer.on && !g.contact && !g.rarena && (g.X.flags & 3) != 3;
 }
 
 // One lane's share of a round.
@@ -474,6 +510,17 @@ C2A_HD void round_lane(Wave<K, CAP>& g, const Leader& L, int lane, int nl, const
   for (int h = lane + 1; h < Shape<K>::NSLOT; h += nl) compute_slot(g, h, M1, M2, contact);
 }
 
+// A round whose slots the scorer orders (round_scored): the slots and their
+// features; the caller then scores them.
+template <int K, int CAP>
+C2A_HD void round_lane_scored(Wave<K, CAP>& g, const Leader& L, int lane, int nl,
+                              const KArgs& a) {
+  const MeshDev M1 = a.meshes[g.mesh[0]];
+  const MeshDev M2 = a.meshes[g.mesh[1]];
+  for (int h = lane + 1; h < Shape<K>::NSLOT; h += nl)
+    compute_slot(g, h, M1, M2, false, &g.fx[h][0], L.Up);
+}
+
 template <int K, int CAP>
 C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
   Entry& e = g.stack[L.sp - 1];
@@ -500,8 +547,8 @@ C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, 
   lane_sync();
   // internal pairs of a CA step go to the arena while it has room
   constexpr int NPAIR = (1 << K) - 1;
-  const bool to_arena = a.arena_rounds && a.logcap > 0 && !g.contact && (flags & 3) != 3 &&
-                        L.napp + NPAIR <= 2 * a.logcap;
+  const bool to_arena = a.arena_rounds && !a.scorer.on && a.logcap > 0 && !g.contact &&
+                        (flags & 3) != 3 && L.napp + NPAIR <= 2 * a.logcap;
   if (lane == 0) {
     if (to_arena) {
       g.rarena = 1;
@@ -574,160 +621,6 @@ C2A_HD void start_traversal(Wave<K, CAP>& g, Leader& L, const MeshDev& M1, const
   L.sp = 1;
 }
 
-// Recomputes, at this step's poses, every pair the previous step's walk
-// computed as a child of an expansion (C2A.cpp:1195-1276 per pair), level by
-// level so that each record's parent transform is ready before it is needed.
-// Lanes take one record each.
-template <int K, int CAP>
-C2A_HD void replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
-  const ArenaView av = arena_view(L.ab, a.logcap);
-  const LogEnt C2A_GLOBAL* lg = gp(L.logbuf ? av.log0 : av.log1);
-  double C2A_GLOBAL* xf = gp(av.xf);
-  int32_t C2A_GLOBAL* cpa = gp(av.cpa);
-  int32_t C2A_GLOBAL* order = gp(av.order);
-  HotRec C2A_GLOBAL* hot = gp(av.hot);
-  const int n = L.nlog_prev;
-  const int nrec = 1 + 2 * n;
-  const MeshDev M1 = a.meshes[g.mesh[0]];
-  const MeshDev M2 = a.meshes[g.mesh[1]];
-  node_ptr N1 = nodes_of(M1);
-  node_ptr N2 = nodes_of(M2);
-  mem_sync();   // the previous walk's log stores
-  // level histogram, children links
-  int32_t C2A_GLOBAL* leafq = gp(av.pos);   // free until the layout pass
-  for (int i = lane; i < CAP + 2; i += nl) g.lh[i] = 0;
-  if (lane == 0) g.nleafq = 0;
-  for (int i = lane; i < nrec; i += nl) cpa[i] = -1;
-  mem_sync();
-  for (int e = lane; e < n; e += nl) lds_add(&g.lh[lg[e].level], 1);
-  for (int e = lane; e < n; e += nl) cpa[lg[e].rec] = e;
-  lane_sync();
-  if (lane == 0) {
-    int acc = 0, top = 0;
-    for (int v = 0; v < CAP + 2; v++) {
-      g.ls[v] = acc;
-      const int c = g.lh[v];
-      if (c) top = v + 1;
-      g.lh[v] = acc;
-      acc += c;
-    }
-    g.ls[CAP + 2] = top;   // number of levels in use
-    // root record: the transform start_traversal placed in the root entry
-    for (int i = 0; i < 9; i++) xf[i] = g.stack[0].R[i];
-    for (int i = 0; i < 3; i++) xf[9 + i] = g.stack[0].T[i];
-  }
-  lane_sync();
-  for (int e = lane; e < n; e += nl) {
-    const int pos = lds_add(&g.lh[lg[e].level], 1);
-    order[pos] = e;
-  }
-  mem_sync();
-  const int top = g.ls[CAP + 2];
-  for (int v = 0; v < top; v++) {
-    const int s0 = g.ls[v];
-    const int cnt = (v + 1 < CAP + 2 ? g.ls[v + 1] : n) - s0;
-    for (int j = lane; j < 2 * cnt; j += nl) {
-      const int e = order[s0 + (j >> 1)];
-      const int side = j & 1;
-      const LogEnt le = lg[e];
-      const int rec = 2 * e + 1 + side;
-      double R[9], T[3];
-      const double C2A_GLOBAL* pxf = xf + 12 * (int64_t)le.rec;
-      for (int i = 0; i < 9; i++) R[i] = pxf[i];
-      for (int i = 0; i < 3; i++) T[i] = pxf[9 + i];
-      int b1 = le.b1, b2 = le.b2;
-      descend(N1, N2, b1, b2, side, R, T);   // an expanded pair is never a leaf pair
-      double d, mt;
-      const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], false, d, mt);
-      double C2A_GLOBAL* oxf = xf + 12 * (int64_t)rec;
-      for (int i = 0; i < 9; i++) oxf[i] = R[i];
-      for (int i = 0; i < 3; i++) oxf[9 + i] = T[i];
-      HotRec h;
-      h.d = d;
-      h.mt = mt;
-      h.b1 = b1;
-      h.b2 = b2;
-      h.flags = fl;
-      h.cp = cpa[rec];
-      h.dTri = 0.0;
-      h.mleaf = 0.0;
-      h.t1 = -1;
-      h.t2 = -1;
-      h.pad[0] = h.pad[1] = 0;
-      hot[rec] = h;
-      if (fl == 3) leafq[lds_add(&g.nleafq, 1)] = rec;
-    }
-    mem_sync();
-  }
-  // triangle distances of the leaf pairs, in dense chunks (C2A.cpp:1141-1166)
-  const int nleaf = g.nleafq;
-  for (int k = lane; k < nleaf; k += nl) {
-    const int rec = leafq[k];
-    HotRec C2A_GLOBAL* hr = hot + rec;
-    Slot tmp;
-    tmp.flags = 3;
-    leaf_values(tmp, M1, M2, g.Rrel, g.Trel, g.r1, g.tt1, g.mo[0], g.mo[1], hr->b1, hr->b2, false);
-    hr->dTri = tmp.dTri;
-    hr->mleaf = tmp.mleaf;
-    hr->t1 = tmp.t1;
-    hr->t2 = tmp.t2;
-  }
-  mem_sync();
-  // Scan layout: replayed subtree sizes bottom-up, then depth-first
-  // positions top-down in this step's visiting order (closer child first,
-  // C2A.cpp:1281), then the records scattered into that order.
-  int32_t C2A_GLOBAL* sz = gp(av.sz);
-  int32_t C2A_GLOBAL* pos = gp(av.pos);
-  Event C2A_GLOBAL* ev = gp(av.ev);
-  for (int i = lane; i < nrec; i += nl) sz[i] = 1;
-  mem_sync();
-  for (int v = top - 1; v >= 0; v--) {
-    const int s0 = g.ls[v];
-    const int cnt = (v + 1 < CAP + 2 ? g.ls[v + 1] : n) - s0;
-    for (int j = lane; j < cnt; j += nl) {
-      const int e = order[s0 + j];
-      sz[lg[e].rec] = 1 + sz[2 * e + 1] + sz[2 * e + 2];
-    }
-    mem_sync();
-  }
-  if (lane == 0) pos[0] = -1;
-  mem_sync();
-  for (int v = 0; v < top; v++) {
-    const int s0 = g.ls[v];
-    const int cnt = (v + 1 < CAP + 2 ? g.ls[v + 1] : n) - s0;
-    for (int j = lane; j < cnt; j += nl) {
-      const int e = order[s0 + j];
-      const int p0 = pos[lg[e].rec] + 1;
-      const int rA = 2 * e + 1, rC = 2 * e + 2;
-      const bool c_first = hot[rC].d < hot[rA].d;
-      const int rf = c_first ? rC : rA, rs = c_first ? rA : rC;
-      pos[rf] = p0;
-      pos[rs] = p0 + sz[rf];
-    }
-    mem_sync();
-  }
-  for (int r = lane + 1; r < nrec; r += nl) {
-    const LogEnt le = lg[(r - 1) >> 1];
-    const HotRec h = hot[r];
-    const bool leaf = (h.flags & 3) == 3;
-    Event E;
-    E.d = h.d;
-    E.mt = h.mt;
-    E.dTri = h.dTri;
-    E.mleaf = h.mleaf;
-    E.b1 = leaf ? h.t1 : h.b1;
-    E.b2 = leaf ? h.t2 : h.b2;
-    E.rec = r;
-    E.size = sz[r];
-    E.ppos = pos[le.rec];
-    E.info = (h.flags & 3) | (h.cp >= 0 ? 4 : 0) | (((r - 1) & 1) << 3) | ((le.level + 1) << 8);
-    ev[pos[r]] = E;
-  }
-  mem_sync();
-  L.wbase = -(1 << 30);
-  lane_sync();
-}
-
 // Wavefront primitives of the scan (the host harness runs one lane).
 C2A_HD uint64_t wave_ballot(bool p) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -794,6 +687,32 @@ C2A_HD int wave_maxi(int v) {
 #endif
   return v;
 }
+C2A_HD float unif(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+#else
+  return x;
+#endif
+}
+C2A_HD unsigned long long global_add_u64(unsigned long long* p, unsigned long long v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicAdd(p, v);
+#else
+  const unsigned long long o = *p;
+  *p = o + v;
+  return o;
+#endif
+}
+C2A_HD long long lane0_bcast64(long long v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const unsigned long long u = (unsigned long long)v;
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+#else
+  return v;
+#endif
+}
 C2A_HD int lds_max(int32_t* p, int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return atomicMax(p, v);
@@ -802,6 +721,222 @@ C2A_HD int lds_max(int32_t* p, int v) {
   if (v > o) *p = v;
   return o;
 #endif
+}
+
+// Training samples of one replayed step: every record below the root whose
+// replayed subtree holds a leaf pair, selected by hashing its key, goes out
+// as its features and log(closest replayed triangle distance + 1e-4).
+// Wavefront-aggregated: one atomic per 64 records.
+template <class LeaderT>
+C2A_HD void write_samples(const LeaderT& L, const KArgs& a, const ArenaView& av, int nrec,
+                          int lane, int nl) {
+  const double C2A_GLOBAL* ml = gp(av.minleaf);
+  const uint16_t C2A_GLOBAL* fe = gp(av.feat);
+  for (int base = 1; base < nrec; base += nl) {
+    const int r = base + lane;
+    const int64_t key = ((int64_t)L.qi << 34) | ((int64_t)(L.numCA & 0xff) << 26) | (r & 0x3ffffff);
+    const bool take = r < nrec && ml[r] < INFINITY && (mix64((uint64_t)key) >> 32) < a.sample_thr;
+    const uint64_t m = wave_ballot(take);
+    if (!m) continue;
+    long long start = 0;
+    if (lane == 0) start = (long long)global_add_u64(a.sample_count, (unsigned long long)__builtin_popcountll(m));
+    start = lane0_bcast64(start);
+    const long long slot = start + lanes_below(m, lane);
+    if (take && slot < a.sample_cap) {
+      float C2A_GLOBAL* o = gp(a.samples) + slot * (NF + 1);
+      for (int k = 0; k < NF; k++) o[k] = bf16_to_f32(fe[(int64_t)r * NF + k]);
+      o[NF] = slog(ml[r]);
+      gp(a.sample_keys)[slot] = key;
+    }
+  }
+}
+
+// Recomputes, at this step's poses, every pair the previous step's walk
+// computed as a child of an expansion (C2A.cpp:1195-1276 per pair), level by
+// level so that each record's parent transform is ready before it is needed.
+// Lanes take one record each.
+template <int K, int CAP>
+C2A_HD void replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
+  const ArenaView av = arena_view(L.ab, a.logcap, a.feats != 0);
+  const LogEnt C2A_GLOBAL* lg = gp(L.logbuf ? av.log0 : av.log1);
+  double C2A_GLOBAL* xf = gp(av.xf);
+  int32_t C2A_GLOBAL* cpa = gp(av.cpa);
+  int32_t C2A_GLOBAL* order = gp(av.order);
+  HotRec C2A_GLOBAL* hot = gp(av.hot);
+  const int n = L.nlog_prev;
+  const int nrec = 1 + 2 * n;
+  const MeshDev M1 = a.meshes[g.mesh[0]];
+  const MeshDev M2 = a.meshes[g.mesh[1]];
+  node_ptr N1 = nodes_of(M1);
+  node_ptr N2 = nodes_of(M2);
+  mem_sync();   // the previous walk's log stores
+  // level histogram, children links
+  int32_t C2A_GLOBAL* leafq = gp(av.pos);   // free until the layout pass
+  for (int i = lane; i < CAP + 2; i += nl) g.lh[i] = 0;
+  if (lane == 0) g.nleafq = 0;
+  for (int i = lane; i < nrec; i += nl) cpa[i] = -1;
+  mem_sync();
+  for (int e = lane; e < n; e += nl) lds_add(&g.lh[lg[e].level], 1);
+  for (int e = lane; e < n; e += nl) cpa[lg[e].rec] = e;
+  lane_sync();
+  if (lane == 0) {
+    int acc = 0, top = 0;
+    for (int v = 0; v < CAP + 2; v++) {
+      g.ls[v] = acc;
+      const int c = g.lh[v];
+      if (c) top = v + 1;
+      g.lh[v] = acc;
+      acc += c;
+    }
+    g.ls[CAP + 2] = top;   // number of levels in use
+    // root record: the transform start_traversal placed in the root entry
+    for (int i = 0; i < 9; i++) xf[i] = g.stack[0].R[i];
+    for (int i = 0; i < 3; i++) xf[9 + i] = g.stack[0].T[i];
+  }
+  lane_sync();
+  for (int e = lane; e < n; e += nl) {
+    const int pos = lds_add(&g.lh[lg[e].level], 1);
+    order[pos] = e;
+  }
+  mem_sync();
+  const int top = g.ls[CAP + 2];
+  for (int v = 0; v < top; v++) {
+    const int s0 = g.ls[v];
+    const int cnt = (v + 1 < CAP + 2 ? g.ls[v + 1] : n) - s0;
+    for (int j = lane; j < 2 * cnt; j += nl) {
+      const int e = order[s0 + (j >> 1)];
+      const int side = j & 1;
+      const LogEnt le = lg[e];
+      const int rec = 2 * e + 1 + side;
+      double R[9], T[3];
+      const double C2A_GLOBAL* pxf = xf + 12 * (int64_t)le.rec;
+      for (int i = 0; i < 9; i++) R[i] = pxf[i];
+      for (int i = 0; i < 3; i++) T[i] = pxf[9 + i];
+      int b1 = le.b1, b2 = le.b2;
+      descend(N1, N2, b1, b2, side, R, T);   // an expanded pair is never a leaf pair
+      double d, mt, mb[2] = {0.0, 0.0};
+      const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], false, d, mt, mb);
+      if (a.feats) {
+        uint16_t f[NF];
+        pair_feature_row(f, N1, N2, b1, b2, R, T, d, mt, mb, g.mo[0], g.mo[1], L.Up, fl);
+        uint16_t C2A_GLOBAL* of = gp(av.feat) + (int64_t)rec * NF;
+        for (int k = 0; k < NF; k++) of[k] = f[k];
+      }
+      double C2A_GLOBAL* oxf = xf + 12 * (int64_t)rec;
+      for (int i = 0; i < 9; i++) oxf[i] = R[i];
+      for (int i = 0; i < 3; i++) oxf[9 + i] = T[i];
+      HotRec h;
+      h.d = d;
+      h.mt = mt;
+      h.b1 = b1;
+      h.b2 = b2;
+      h.flags = fl;
+      h.cp = cpa[rec];
+      h.dTri = 0.0;
+      h.mleaf = 0.0;
+      h.t1 = -1;
+      h.t2 = -1;
+      h.score = 0.0f;
+      h.pad = 0;
+      hot[rec] = h;
+      if (fl == 3) leafq[lds_add(&g.nleafq, 1)] = rec;
+    }
+    mem_sync();
+  }
+  // triangle distances of the leaf pairs, in dense chunks (C2A.cpp:1141-1166)
+  const int nleaf = g.nleafq;
+  for (int k = lane; k < nleaf; k += nl) {
+    const int rec = leafq[k];
+    HotRec C2A_GLOBAL* hr = hot + rec;
+    Slot tmp;
+    tmp.flags = 3;
+    leaf_values(tmp, M1, M2, g.Rrel, g.Trel, g.r1, g.tt1, g.mo[0], g.mo[1], hr->b1, hr->b2, false);
+    hr->dTri = tmp.dTri;
+    hr->mleaf = tmp.mleaf;
+    hr->t1 = tmp.t1;
+    hr->t2 = tmp.t2;
+  }
+  mem_sync();
+}
+
+// Scan layout of the replayed records: replayed subtree sizes bottom-up,
+// then depth-first positions top-down in this step's visiting order (closer
+// child first, C2A.cpp:1281, or the lower score under the learned order),
+// then the records scattered into that order.  Training samples of the
+// replayed pairs are written on the way up.
+template <int K, int CAP>
+C2A_HD void replay_layout(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
+  const ArenaView av = arena_view(L.ab, a.logcap, a.feats != 0);
+  const LogEnt C2A_GLOBAL* lg = gp(L.logbuf ? av.log0 : av.log1);
+  int32_t C2A_GLOBAL* order = gp(av.order);
+  HotRec C2A_GLOBAL* hot = gp(av.hot);
+  const int n = L.nlog_prev;
+  const int nrec = 1 + 2 * n;
+  const int top = g.ls[CAP + 2];
+  const bool learned = a.scorer.on != 0;
+  const bool sampling = a.sample_thr != 0;
+  int32_t C2A_GLOBAL* sz = gp(av.sz);
+  int32_t C2A_GLOBAL* pos = gp(av.pos);
+  Event C2A_GLOBAL* ev = gp(av.ev);
+  double C2A_GLOBAL* ml = gp(av.minleaf);
+  for (int i = lane; i < nrec; i += nl) {
+    sz[i] = 1;
+    if (sampling) {
+      const HotRec h = hot[i];
+      ml[i] = (i > 0 && (h.flags & 3) == 3) ? h.dTri : INFINITY;
+    }
+  }
+  mem_sync();
+  for (int v = top - 1; v >= 0; v--) {
+    const int s0 = g.ls[v];
+    const int cnt = (v + 1 < CAP + 2 ? g.ls[v + 1] : n) - s0;
+    for (int j = lane; j < cnt; j += nl) {
+      const int e = order[s0 + j];
+      sz[lg[e].rec] = 1 + sz[2 * e + 1] + sz[2 * e + 2];
+      if (sampling) {
+        const double u = ml[2 * e + 1], w = ml[2 * e + 2];
+        ml[lg[e].rec] = u < w ? u : w;
+      }
+    }
+    mem_sync();
+  }
+  if (sampling) write_samples(L, a, av, nrec, lane, nl);
+  if (lane == 0) pos[0] = -1;
+  mem_sync();
+  for (int v = 0; v < top; v++) {
+    const int s0 = g.ls[v];
+    const int cnt = (v + 1 < CAP + 2 ? g.ls[v + 1] : n) - s0;
+    for (int j = lane; j < cnt; j += nl) {
+      const int e = order[s0 + j];
+      const int p0 = pos[lg[e].rec] + 1;
+      const int rA = 2 * e + 1, rC = 2 * e + 2;
+      const bool c_first = learned ? hot[rC].score < hot[rA].score : hot[rC].d < hot[rA].d;
+      const int rf = c_first ? rC : rA, rs = c_first ? rA : rC;
+      pos[rf] = p0;
+      pos[rs] = p0 + sz[rf];
+    }
+    mem_sync();
+  }
+  for (int r = lane + 1; r < nrec; r += nl) {
+    const LogEnt le = lg[(r - 1) >> 1];
+    const HotRec h = hot[r];
+    const bool leaf = (h.flags & 3) == 3;
+    Event E;
+    E.d = h.d;
+    E.mt = h.mt;
+    E.dTri = h.dTri;
+    E.mleaf = h.mleaf;
+    E.b1 = leaf ? h.t1 : h.b1;
+    E.b2 = leaf ? h.t2 : h.b2;
+    E.rec = r;
+    E.size = sz[r];
+    E.ppos = pos[le.rec];
+    E.info = (h.flags & 3) | (h.cp >= 0 ? 4 : 0) | (((r - 1) & 1) << 3) | ((le.level + 1) << 8);
+    ev[pos[r]] = E;
+  }
+  mem_sync();
+  L.wbase = -(1 << 30);
+  lane_sync();
 }
 
 // Starts the scan of a replayed step: the root pair is always expanded
@@ -1009,6 +1144,7 @@ C2A_HD void push_back(Wave<K, CAP>& g, Leader& L, const Cur& c, int where, int c
 template <int K, int CAP>
 C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
   const bool contact = g.contact != 0;
+  const bool learned = a.scorer.on && !contact;
   const int gen = uni(g.gen);
   const ArenaView av = arena_view(L.ab, a.logcap);
   const int nrec_max = arena_records(a.logcap);
@@ -1111,8 +1247,11 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
       C2A_CHECK(ra - L.wbase >= 0 && ra + 1 - L.wbase < WIN && L.sp < CAP);
       const Cur cA = cur_from_hot(g.win[ra - L.wbase], ra, nnA, clvl);
       const Cur cC = cur_from_hot(g.win[ra + 1 - L.wbase], ra + 1, nnC, clvl);
-      // closer child visited first (C2A.cpp:1281); the other one waits
-      const bool c_first = cC.d < cA.d;
+      // closer child visited first (C2A.cpp:1281), or the lower score under
+      // the learned order; the other one waits
+      const bool c_first = learned ? unif(g.win[ra + 1 - L.wbase].score) <
+                                         unif(g.win[ra - L.wbase].score)
+                                   : cC.d < cA.d;
       lane_sync();
       write_cur(g.stack[L.sp], c_first ? cA : cC, lane);
       lane_sync();
@@ -1124,7 +1263,8 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
     }
     const int sa = 2 * s + 1, sc = 2 * s + 2;
     const double dA = unid(g.slot[sa].d), dC = unid(g.slot[sc].d);
-    const bool c_first = dC < dA;
+    const bool c_first =
+        learned ? unif(g.slot[sc].score) < unif(g.slot[sa].score) : dC < dA;
     const int sf = c_first ? sc : sa, ss = c_first ? sa : sc;
     const int nf = c_first ? nnC : nnA, ns = c_first ? nnA : nnC;
     const Slot& F = g.slot[sf];
@@ -1442,6 +1582,7 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
 template <int K, int CAP>
 C2A_HD void leader_run(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
   L.need_round = 0;
+  L.need_score = 0;
   for (;;) {
     switch (L.state) {
       case ST_FETCH:
@@ -1451,11 +1592,19 @@ C2A_HD void leader_run(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int
         begin_step(g, L, a, lane);
         if (a.logcap > 0 && L.nlog_prev > 0) {
           replay(g, L, a, lane, nl);
-          start_scan(g, L, a, lane, nl);
-          L.state = ST_SCAN;
+          L.state = ST_LAYOUT;
+          if (a.scorer.on) {   // the caller scores the records, then comes back
+            L.need_score = 1;
+            return;
+          }
         } else {
           L.state = ST_WALK;
         }
+        break;
+      case ST_LAYOUT:
+        replay_layout(g, L, a, lane, nl);
+        start_scan(g, L, a, lane, nl);
+        L.state = ST_SCAN;
         break;
       case ST_SCAN:
         if (scan(g, L, a, lane, nl)) {

@@ -14,10 +14,25 @@
 #include "bvh.h"
 #include "ccd_kernel.h"
 #include "ccd_core.h"
+#include "mlp_dev.h"
 
 namespace c2a {
 
-template <int K, int CAP, int NW>
+// Learned order: scores of the replayed records (features in the arena,
+// 64 records per matrix-core pass); the layout then orders siblings by them.
+template <int K, int CAP>
+__device__ void score_replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
+  const ArenaView av = arena_view(L.ab, a.logcap, true);
+  const int nrec = 1 + 2 * L.nlog_prev;
+  for (int base = 1; base < nrec; base += 64) {
+    // rows up to base + 63 stay inside the feature array's 64-row slack
+    const float y = score64(a.scorer, av.feat + (int64_t)base * NF, lane);
+    if (base + lane < nrec) gp(av.hot)[base + lane].score = y;
+  }
+  mem_sync();
+}
+
+template <int K, int CAP, int NW, bool LRN>
 __global__ void __launch_bounds__(64 * NW) ccd_kernel(KArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Wave<K, CAP>& g = *reinterpret_cast<Wave<K, CAP>*>(smem_raw);
@@ -25,10 +40,25 @@ __global__ void __launch_bounds__(64 * NW) ccd_kernel(KArgs a) {
   Leader L;
   L.state = ST_FETCH;
   L.sp = 0;
-  L.ab = a.logcap > 0 ? a.arena + (size_t)blockIdx.x * arena_bytes(a.logcap) : nullptr;
+  L.ab = a.logcap > 0 ? a.arena + (size_t)blockIdx.x * arena_bytes(a.logcap, a.feats != 0)
+                      : nullptr;
   for (;;) {
     leader_run(g, L, a, lane, 64 * NW);
     if (L.state == ST_DONE) break;
+    if constexpr (LRN) {
+      if (L.need_score) {
+        score_replay(g, L, a, lane);
+        continue;
+      }
+      if (round_scored(g, a)) {
+        round_lane_scored(g, L, lane, 64 * NW, a);
+        lane_sync();
+        const float y = score64(a.scorer, &g.fx[0][0], lane);
+        if (lane >= 1 && lane < Shape<K>::NSLOT) g.slot[lane].score = y;
+        lane_sync();
+        continue;
+      }
+    }
     round_lane(g, L, lane, 64 * NW, a);
     if (g.rarena) mem_sync();   // the walk reads the new records through the LDS window
     else lane_sync();
@@ -38,10 +68,10 @@ __global__ void __launch_bounds__(64 * NW) ccd_kernel(KArgs a) {
 // ---------------------------------------------------------------------------
 // host-side launch
 // ---------------------------------------------------------------------------
-template <int K, int CAP, int NW>
+template <int K, int CAP, int NW, bool LRN>
 static hipError_t launch_one(const KArgs& a, int blocks, hipStream_t s) {
   const size_t lds = sizeof(Wave<K, CAP>);
-  auto fn = ccd_kernel<K, CAP, NW>;
+  auto fn = ccd_kernel<K, CAP, NW, LRN>;
   hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)lds);
   if (e != hipSuccess) return e;
@@ -49,10 +79,10 @@ static hipError_t launch_one(const KArgs& a, int blocks, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int K, int CAP, int NW>
+template <int K, int CAP, int NW, bool LRN>
 static int occupancy_one() {
   const size_t lds = sizeof(Wave<K, CAP>);
-  auto fn = ccd_kernel<K, CAP, NW>;
+  auto fn = ccd_kernel<K, CAP, NW, LRN>;
   if (hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
       hipSuccess)
     return 0;
@@ -65,37 +95,42 @@ static int occupancy_one() {
 // supports NW > 1 (the replay then uses every wavefront), but a second
 // instantiation of the same K/CAP makes the compiler outline the shared
 // per-query code, which costs more than the extra wavefronts gain.
+// The learned order (LRN) is instantiated at the default depth only.
 #define C2A_SHAPES(X)                                                                  \
-  X(1, 48, 1) X(2, 48, 1) X(3, 48, 1) X(4, 48, 1) X(5, 48, 1) X(6, 48, 1)              \
-  X(1, 96, 1) X(2, 96, 1) X(3, 96, 1) X(4, 96, 1) X(5, 96, 1) X(6, 96, 1)
+  X(1, 48, 1, 0) X(2, 48, 1, 0) X(3, 48, 1, 0) X(4, 48, 1, 0) X(5, 48, 1, 0)            \
+  X(6, 48, 1, 0) X(1, 96, 1, 0) X(2, 96, 1, 0) X(3, 96, 1, 0) X(4, 96, 1, 0)            \
+  X(5, 96, 1, 0) X(6, 96, 1, 0) X(5, 48, 1, 1) X(5, 96, 1, 1)
 
-bool kernel_shape_ok(int K, int CAP, int NW) {
-#define C2A_OK_CASE(k, cap, nw) \
-  if (K == k && CAP == cap && NW == nw) return true;
+bool kernel_shape_ok(int K, int CAP, int NW, bool LRN) {
+#define C2A_OK_CASE(k, cap, nw, lrn) \
+  if (K == k && CAP == cap && NW == nw && LRN == (lrn != 0)) return true;
   C2A_SHAPES(C2A_OK_CASE)
 #undef C2A_OK_CASE
   return false;
 }
 
 size_t kernel_lds_bytes(int K, int CAP) {
-#define C2A_LDS_CASE(k, cap, nw) \
+#define C2A_LDS_CASE(k, cap, nw, lrn) \
   if (K == k && CAP == cap) return sizeof(Wave<k, cap>);
   C2A_SHAPES(C2A_LDS_CASE)
 #undef C2A_LDS_CASE
   return 0;
 }
 
-int kernel_blocks_per_cu(int K, int CAP, int NW) {
-#define C2A_OCC_CASE(k, cap, nw) \
-  if (K == k && CAP == cap && NW == nw) return occupancy_one<k, cap, nw>();
+int kernel_blocks_per_cu(int K, int CAP, int NW, bool LRN) {
+#define C2A_OCC_CASE(k, cap, nw, lrn) \
+  if (K == k && CAP == cap && NW == nw && LRN == (lrn != 0)) \
+    return occupancy_one<k, cap, nw, (lrn != 0)>();
   C2A_SHAPES(C2A_OCC_CASE)
 #undef C2A_OCC_CASE
   return 0;
 }
 
 hipError_t launch_ccd(const KArgs& a, int K, int CAP, int NW, int blocks, hipStream_t s) {
-#define C2A_LAUNCH_CASE(k, cap, nw) \
-  if (K == k && CAP == cap && NW == nw) return launch_one<k, cap, nw>(a, blocks, s);
+  const bool LRN = a.scorer.on != 0;
+#define C2A_LAUNCH_CASE(k, cap, nw, lrn)                       \
+  if (K == k && CAP == cap && NW == nw && LRN == (lrn != 0)) \
+    return launch_one<k, cap, nw, (lrn != 0)>(a, blocks, s);
   C2A_SHAPES(C2A_LAUNCH_CASE)
 #undef C2A_LAUNCH_CASE
   return hipErrorInvalidValue;

@@ -7,6 +7,7 @@
 #include "../../include/c2a_amd.h"
 #include "bvh.h"
 #include "c2a_math.h"
+#include "scorer.h"
 
 namespace c2a {
 
@@ -65,7 +66,8 @@ struct HotRec {          // what the walk reads of a replayed record (64 B)
   int32_t flags;         // bit0 leaf node of model 1, bit1 leaf node of model 2
   int32_t cp;            // expansion whose children are this pair's children, -1 none
   int32_t t1, t2;        // leaf pairs: triangle indices
-  int32_t pad[2];
+  float score;           // learned order: the scorer's output for the pair
+  int32_t pad;
 };
 
 // One replayed pair as the lane-parallel scan reads it: the records of the
@@ -92,22 +94,27 @@ struct ArenaView {
   int32_t* sz;           // nrec: replayed subtree size of each record
   int32_t* pos;          // nrec: scan position of each record
   Event* ev;             // nrec - 1: the scan order
+  // learned ordering / training samples (only when the arena has room for them)
+  uint16_t* feat;        // nrec x NF pair features (BF16)
+  double* minleaf;       // nrec closest replayed triangle distance in the subtree
 };
 
 // Records: the replayed ones [0, 1 + 2 * logcap), then room for as many
 // again computed by speculative rounds during the step.
 C2A_HD int arena_records(int logcap) { return 1 + 4 * logcap; }
 
-C2A_HD size_t arena_bytes(int logcap) {
+C2A_HD size_t arena_bytes(int logcap, bool feats = false) {
   const size_t nrec = 1 + 2 * (size_t)logcap;
   const size_t nall = (size_t)arena_records(logcap);
   size_t b = 2 * (size_t)logcap * sizeof(LogEnt) + nall * 12 * sizeof(double) +
              nall * sizeof(HotRec) + nrec * sizeof(int32_t) + (size_t)logcap * sizeof(int32_t) +
              2 * nrec * sizeof(int32_t) + nrec * sizeof(Event);
+  // features: 64 rows of slack so that the scorer may read whole 64-row tiles
+  if (feats) b += 16 + nrec * sizeof(double) + (nrec + 64) * NF * sizeof(uint16_t);
   return (b + 16 + 255) & ~(size_t)255;
 }
 
-C2A_HD ArenaView arena_view(unsigned char* base, int logcap) {
+C2A_HD ArenaView arena_view(unsigned char* base, int logcap, bool feats = false) {
   const size_t nrec = 1 + 2 * (size_t)logcap;
   const size_t nall = (size_t)arena_records(logcap);
   ArenaView v;
@@ -130,6 +137,15 @@ C2A_HD ArenaView arena_view(unsigned char* base, int logcap) {
   p += nrec * sizeof(int32_t);
   p = (unsigned char*)(((uintptr_t)p + 15) & ~(uintptr_t)15);
   v.ev = (Event*)p;
+  p += nrec * sizeof(Event);
+  v.feat = nullptr;
+  v.minleaf = nullptr;
+  if (feats) {
+    p = (unsigned char*)(((uintptr_t)p + 15) & ~(uintptr_t)15);
+    v.minleaf = (double*)p;
+    p += nrec * sizeof(double);
+    v.feat = (uint16_t*)p;
+  }
   return v;
 }
 
@@ -150,7 +166,25 @@ struct KArgs {
   int32_t mode;
   int32_t logcap;          // expansions logged per step per wavefront, 0 = no replay
   int32_t arena_rounds;    // speculative rounds write their subtree to the arena
-  unsigned char* arena;    // one arena_bytes(logcap) slice per workgroup
+  unsigned char* arena;    // one arena_bytes(logcap, feats) slice per workgroup
+  int32_t feats;           // the arena holds features (learned order or samples)
+  int32_t pad2;
+  ScorerDev scorer;        // learned child order when scorer.on
+  // training samples of replayed pairs: a pair is logged when the top 32
+  // bits of mix64(key) are below sample_thr (2^32 = every pair)
+  uint64_t sample_thr;
+  float* samples;          // sample_cap x (NF + 1): features, then the label
+  int64_t* sample_keys;    // sample_cap: query << 34 | CA step << 26 | record
+  int64_t sample_cap;
+  unsigned long long* sample_count;   // rows claimed (may exceed sample_cap)
 };
+
+// splitmix64 finaliser: the sample selection hash.
+C2A_HD uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
 
 }  // namespace c2a

@@ -113,6 +113,54 @@ __device__ inline void mlp_forward64(const ScorerDev& S, MlpLds& m, int lane, f3
   wave_lds_sync();
 }
 
+__device__ inline bf16x4 ld_frag(const uint16_t* p) {
+  // four consecutive BF16 values (8-byte aligned)
+  const uint64_t u = *reinterpret_cast<const uint64_t*>(p);
+  return __builtin_bit_cast(bf16x4, u);
+}
+
+// Scores of 64 BF16 feature rows X[64][NF] (row-major, any address space),
+// computed transposed so that nothing leaves the registers:
+// H1^T = W1^T X^T and H2^T = W2^T H1^T.  The accumulator of H1^T already
+// holds, lane by lane, the B operand of the second product (k = hidden unit
+// 8 ks + 4 (lane / 32) + i sits in register 4 (ks % 4) + i of row tile
+// ks / 4).  Returns the score of row `lane`.  All 64 lanes must be active.
+__device__ inline float score64(const ScorerDev& S, const uint16_t* X, int lane) {
+  const int hi = lane >> 5, lo = lane & 31;
+  float y[2];
+  for (int nt = 0; nt < 2; nt++) {   // feature rows 32 nt .. 32 nt + 31
+    f32x16 h1[2];
+    for (int mt = 0; mt < 2; mt++) {
+      f32x16 acc = {0};
+      for (int ks = 0; ks < NF / 8; ks++)
+        acc = mfma(ld_frag(S.w1t + (32 * mt + lo) * NF + 8 * ks + 4 * hi),
+                   ld_frag(X + (32 * nt + lo) * NF + 8 * ks + 4 * hi), acc);
+      h1[mt] = acc;
+    }
+    bf16x4 hb[NH / 8];
+    for (int ks = 0; ks < NH / 8; ks++)
+      for (int i = 0; i < 4; i++) {
+        const int unit = 8 * ks + 4 * hi + i;
+        const float v = h1[ks >> 2][4 * (ks & 3) + i] + S.b1[unit];
+        hb[ks][i] = (short)f32_to_bf16(v > 0.0f ? v : 0.0f);
+      }
+    float part = 0.0f;
+    for (int mt = 0; mt < 2; mt++) {
+      f32x16 acc = {0};
+      for (int ks = 0; ks < NH / 8; ks++)
+        acc = mfma(ld_frag(S.w2t + (32 * mt + lo) * NH + 8 * ks + 4 * hi), hb[ks], acc);
+      for (int r = 0; r < 16; r++) {
+        const int unit = 32 * mt + crow(r, lane);
+        const float v = acc[r] + S.b2[unit];
+        part += (v > 0.0f ? v : 0.0f) * S.w3[unit];
+      }
+    }
+    part += __shfl_xor(part, 32);
+    y[nt] = part + S.b3;
+  }
+  return hi ? y[1] : y[0];
+}
+
 __device__ inline void grad_zero(MlpGrad& G) {
   const f32x16 z = {0};
   G.w1[0] = G.w1[1] = z;

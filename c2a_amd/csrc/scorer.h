@@ -23,10 +23,13 @@ constexpr int P_W3 = P_B2 + NH;
 constexpr int P_B3 = P_W3 + NH;
 constexpr int NPARAM = P_B3 + 1;   // 6337
 
-// Device copy of a scorer: BF16 weights (row-major [in][out]) and FP32 biases.
+// Device copy of a scorer: BF16 weights (row-major [in][out], and transposed
+// [out][in] for the traversal's register-resident pass) and FP32 biases.
 struct ScorerDev {
   const uint16_t* w1;   // NF x NH
   const uint16_t* w2;   // NH x NH
+  const uint16_t* w1t;  // NH x NF
+  const uint16_t* w2t;  // NH x NH
   const float* b1;
   const float* b2;
   const float* w3;
@@ -105,6 +108,26 @@ inline float score_scalar(const float* x, const float* p) {
   }
   float y = p[P_B3];
   for (int j = 0; j < NH; j++) y += h2[j] * p[P_W3 + j];
+  return y;
+}
+
+// Scalar forward pass over a BF16 feature row and the BF16 transposed weights
+// of a ScorerDev whose arrays are host memory (the host test harness).
+inline float score_dev_scalar(const ScorerDev& S, const uint16_t* xb) {
+  float h1[NH];
+  for (int j = 0; j < NH; j++) {
+    float s = 0.0f;
+    for (int k = 0; k < NF; k++) s += bf16_to_f32(xb[k]) * bf16_to_f32(S.w1t[j * NF + k]);
+    s += S.b1[j];
+    h1[j] = bf16_to_f32(f32_to_bf16(s > 0.0f ? s : 0.0f));
+  }
+  float y = S.b3;
+  for (int j = 0; j < NH; j++) {
+    float s = 0.0f;
+    for (int k = 0; k < NH; k++) s += h1[k] * bf16_to_f32(S.w2t[j * NH + k]);
+    s += S.b2[j];
+    y += (s > 0.0f ? s : 0.0f) * S.w3[j];
+  }
   return y;
 }
 

@@ -7,6 +7,9 @@
 #include <math.h>
 #include <string.h>
 
+#include <stdio.h>
+
+#include <cmath>
 #include <new>
 #include <vector>
 
@@ -16,7 +19,24 @@
 
 namespace c2a {
 
-// Scores of n feature rows (FP32 in, one wavefront per 64 rows).
+// Scores of n feature rows as the traversal computes them (score64 over BF16
+// rows staged in LDS, one wavefront per 64 rows).
+__global__ void __launch_bounds__(64) infer_kernel(ScorerDev S, const float* X, int64_t n,
+                                                   float* out) {
+  __shared__ __align__(16) uint16_t x[64][NF];
+  const int lane = threadIdx.x;
+  for (int64_t base = (int64_t)blockIdx.x * 64; base < n; base += (int64_t)gridDim.x * 64) {
+    const int64_t row = base + lane;
+    for (int k = 0; k < NF; k++) x[lane][k] = f32_to_bf16(row < n ? X[row * NF + k] : 0.0f);
+    wave_lds_sync();
+    const float y = score64(S, &x[0][0], lane);
+    if (row < n) out[row] = y;
+    wave_lds_sync();
+  }
+}
+
+// Scores of n feature rows through the training forward pass (FP32 in, one
+// wavefront per 64 rows).
 __global__ void __launch_bounds__(64) score_rows_kernel(ScorerDev S, const float* X, int64_t n,
                                                         float* out) {
   __shared__ MlpLds m;
@@ -93,7 +113,7 @@ __global__ void reduce_kernel(const float* partial, int nblocks, float* grad) {
 // Adam step on the FP32 master parameters; refreshes the BF16 weights.
 __global__ void adam_kernel(float* p, float* mom, float* vel, const float* grad, float lr,
                             float b1, float b2, float eps, int step, uint16_t* w1,
-                            uint16_t* w2) {
+                            uint16_t* w2, uint16_t* w1t, uint16_t* w2t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= NPARAM) return;
   const float g = grad[i];
@@ -105,14 +125,21 @@ __global__ void adam_kernel(float* p, float* mom, float* vel, const float* grad,
   const float vh = vi / (1.0f - powf(b2, (float)step));
   const float pi = p[i] - lr * mh / (sqrtf(vh) + eps);
   p[i] = pi;
-  if (i >= P_W1 && i < P_B1) w1[i - P_W1] = f32_to_bf16(pi);
-  if (i >= P_W2 && i < P_B2) w2[i - P_W2] = f32_to_bf16(pi);
+  if (i >= P_W1 && i < P_B1) {
+    const int k = i - P_W1;
+    w1[k] = w1t[(k % NH) * NF + k / NH] = f32_to_bf16(pi);
+  }
+  if (i >= P_W2 && i < P_B2) {
+    const int k = i - P_W2;
+    w2[k] = w2t[(k % NH) * NH + k / NH] = f32_to_bf16(pi);
+  }
 }
 
-__global__ void to_bf16_kernel(const float* p, uint16_t* w1, uint16_t* w2) {
+__global__ void to_bf16_kernel(const float* p, uint16_t* w1, uint16_t* w2, uint16_t* w1t,
+                               uint16_t* w2t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < NF * NH) w1[i] = f32_to_bf16(p[P_W1 + i]);
-  if (i < NH * NH) w2[i] = f32_to_bf16(p[P_W2 + i]);
+  if (i < NF * NH) w1[i] = w1t[(i % NH) * NF + i / NH] = f32_to_bf16(p[P_W1 + i]);
+  if (i < NH * NH) w2[i] = w2t[(i % NH) * NH + i / NH] = f32_to_bf16(p[P_W2 + i]);
 }
 
 }  // namespace c2a
@@ -127,6 +154,8 @@ struct c2a_scorer {
   float* vel = nullptr;
   uint16_t* w1 = nullptr;
   uint16_t* w2 = nullptr;
+  uint16_t* w1t = nullptr;
+  uint16_t* w2t = nullptr;
   int step = 0;
   ~c2a_scorer() {
     if (p) (void)hipFree(p);
@@ -134,6 +163,8 @@ struct c2a_scorer {
     if (vel) (void)hipFree(vel);
     if (w1) (void)hipFree(w1);
     if (w2) (void)hipFree(w2);
+    if (w1t) (void)hipFree(w1t);
+    if (w2t) (void)hipFree(w2t);
   }
 };
 
@@ -145,6 +176,8 @@ ScorerDev dev_view(const c2a_scorer* s, float b3) {
   ScorerDev d;
   d.w1 = s->w1;
   d.w2 = s->w2;
+  d.w1t = s->w1t;
+  d.w2t = s->w2t;
   d.b1 = s->p + P_B1;
   d.b2 = s->p + P_B2;
   d.w3 = s->p + P_W3;
@@ -169,6 +202,12 @@ uint64_t mix(uint64_t z) {
 
 }  // namespace
 
+namespace c2a {
+
+ScorerDev scorer_view(const c2a_scorer* s) { return dev_view(s, read_b3(s)); }
+
+}  // namespace c2a
+
 extern "C" {
 
 int c2a_scorer_create(uint64_t seed, c2a_scorer** out) {
@@ -182,7 +221,9 @@ int c2a_scorer_create(uint64_t seed, c2a_scorer** out) {
   if (hipMalloc(&s->p, pb) != hipSuccess || hipMalloc(&s->mom, pb) != hipSuccess ||
       hipMalloc(&s->vel, pb) != hipSuccess ||
       hipMalloc(&s->w1, NF * NH * sizeof(uint16_t)) != hipSuccess ||
-      hipMalloc(&s->w2, NH * NH * sizeof(uint16_t)) != hipSuccess) {
+      hipMalloc(&s->w2, NH * NH * sizeof(uint16_t)) != hipSuccess ||
+      hipMalloc(&s->w1t, NF * NH * sizeof(uint16_t)) != hipSuccess ||
+      hipMalloc(&s->w2t, NH * NH * sizeof(uint16_t)) != hipSuccess) {
     delete s;
     return C2A_ERR_OOM;
   }
@@ -218,13 +259,55 @@ int c2a_scorer_set_params(c2a_scorer* s, const float* params) {
   (void)hipMemset(s->vel, 0, (NPARAM + 1) * sizeof(float));
   s->step = 0;
   hipLaunchKernelGGL(to_bf16_kernel, dim3((NH * NH + 255) / 256), dim3(256), 0, 0, s->p, s->w1,
-                     s->w2);
+                     s->w2, s->w1t, s->w2t);
   return hip_err(hipDeviceSynchronize());
 }
 
 int c2a_scorer_get_params(const c2a_scorer* s, float* params) {
   if (!s || !params) return C2A_ERR_BAD_ARG;
   return hip_err(hipMemcpy(params, s->p, NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+// File format: the 8 bytes "C2ASCR01", the parameter count as a
+// little-endian uint32, then the FP32 parameters (little-endian).
+int c2a_scorer_save(const c2a_scorer* s, const char* path) {
+  if (!s || !path) return C2A_ERR_BAD_ARG;
+  std::vector<float> h(NPARAM);
+  int rc = c2a_scorer_get_params(s, h.data());
+  if (rc) return rc;
+  FILE* f = fopen(path, "wb");
+  if (!f) return C2A_ERR_BAD_ARG;
+  const uint32_t n = NPARAM;
+  const bool ok = fwrite("C2ASCR01", 1, 8, f) == 8 && fwrite(&n, 4, 1, f) == 1 &&
+                  fwrite(h.data(), sizeof(float), NPARAM, f) == (size_t)NPARAM;
+  return (fclose(f) == 0 && ok) ? C2A_OK : C2A_ERR_BAD_ARG;
+}
+
+int c2a_scorer_load(const char* path, c2a_scorer** out) {
+  if (!path || !out) return C2A_ERR_BAD_ARG;
+  *out = nullptr;
+  FILE* f = fopen(path, "rb");
+  if (!f) return C2A_ERR_BAD_ARG;
+  char magic[8];
+  uint32_t n = 0;
+  std::vector<float> h(NPARAM);
+  const bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "C2ASCR01", 8) == 0 &&
+                  fread(&n, 4, 1, f) == 1 && n == (uint32_t)NPARAM &&
+                  fread(h.data(), sizeof(float), NPARAM, f) == (size_t)NPARAM;
+  fclose(f);
+  if (!ok) return C2A_ERR_BAD_ARG;
+  for (float v : h)
+    if (!std::isfinite(v)) return C2A_ERR_BAD_ARG;
+  c2a_scorer* s = nullptr;
+  int rc = c2a_scorer_create(0, &s);
+  if (rc) return rc;
+  rc = c2a_scorer_set_params(s, h.data());
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return C2A_OK;
 }
 
 int c2a_scorer_infer(const c2a_scorer* s, const float* feats, int64_t n, float* out) {
@@ -239,8 +322,8 @@ int c2a_scorer_infer(const c2a_scorer* s, const float* feats, int64_t n, float* 
   hipError_t e = hipMemcpy(dx, feats, n * NF * sizeof(float), hipMemcpyHostToDevice);
   const int grid = (int)((n + 63) / 64 < 4096 ? (n + 63) / 64 : 4096);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(score_rows_kernel, dim3(grid), dim3(64), 0, 0, dev_view(s, read_b3(s)), dx,
-                       n, dy);
+    hipLaunchKernelGGL(infer_kernel, dim3(grid), dim3(64), 0, 0, dev_view(s, read_b3(s)), dx, n,
+                       dy);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(out, dy, n * sizeof(float), hipMemcpyDeviceToHost);
@@ -320,7 +403,7 @@ int c2a_scorer_train(c2a_scorer* s, const float* feats, const float* labels, int
       hipLaunchKernelGGL(reduce_kernel, dim3((NPARAM + 256) / 256), dim3(256), 0, 0, part, nb, g);
       s->step++;
       hipLaunchKernelGGL(adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, 0, s->p, s->mom,
-                         s->vel, g, lr, 0.9f, 0.999f, 1e-8f, s->step, s->w1, s->w2);
+                         s->vel, g, lr, 0.9f, 0.999f, 1e-8f, s->step, s->w1, s->w2, s->w1t, s->w2t);
       e = hipGetLastError();
       float l = 0.0f;
       if (e == hipSuccess) e = hipMemcpy(&l, g + NPARAM, sizeof(float), hipMemcpyDeviceToHost);

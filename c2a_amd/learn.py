@@ -1,0 +1,130 @@
+"""Learned front ordering: training data from the exact GPU traversal, the
+scorer's training on the GPU, and its evaluation.
+
+The data come from the plain (reference-order) traversal itself: every CA
+step after the first replays the previous step's traversal, and each
+replayed pair whose replayed subtree holds a leaf pair becomes a sample --
+its 32 features (c2a_amd/csrc/scorer.h) and the label
+log(closest replayed triangle distance in that subtree + 1e-4).  Splits are
+by query, by hashing the query index with a seed, so no query contributes to
+both sides.  Training runs the project's own matrix-core forward/backward
+kernels (c2a_scorer_train).  The scorer only orders siblings; it never
+prunes a pair, so every result stays a conservative-advancement result.
+
+    python -m c2a_amd.learn --out c2a_amd/scorer_knot_uneven.c2as
+"""
+import argparse
+import os
+import time
+
+import numpy as np
+
+import c2a_amd as G
+from c2a_amd import workloads as W
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_SCORER = os.path.join(_HERE, "scorer_knot_uneven.c2as")
+
+
+def collect_samples(meshes, queries, tol_d, tol_t, rate, cap):
+    """One plain-order batch run with sample logging: (X, y, keys, selected)."""
+    p = G.default_params(tol_d=tol_d, tol_t=tol_t, with_contacts=0, sample_rate=rate,
+                         sample_cap=cap)
+    b = G.Batch(meshes, queries, p)
+    b.run()
+    return b.samples(cap)
+
+
+def query_of(keys):
+    return keys >> 34
+
+
+def split_mask(keys, train_frac, seed):
+    """True for samples of training queries (a seeded hash of the query index)."""
+    q = query_of(keys).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        h = (q + np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15)) * np.uint64(0xBF58476D1CE4E5B9)
+        h ^= h >> np.uint64(31)
+    return (h % np.uint64(1000)) < np.uint64(int(train_frac * 1000))
+
+
+def query_split(nq, train_frac, seed):
+    """The same split over query indices 0..nq-1."""
+    return split_mask(np.arange(nq, dtype=np.int64) << 34, train_frac, seed)
+
+
+def sibling_accuracy(keys, labels, scores):
+    """Fraction of sibling pairs (records 2e+1, 2e+2 of one step) whose order
+    by score puts the child with the closer replayed leaf first, and the same
+    for the plain order's RSS distance (feature 18 = log d) if scores is None."""
+    o = np.argsort(keys, kind="stable")
+    k, y, s = keys[o], labels[o], scores[o]
+    first = (k & 1) == 1            # record 2e+1
+    nxt = np.zeros_like(first)
+    nxt[:-1] = k[1:] == k[:-1] + 1
+    i = np.nonzero(first & nxt)[0]
+    if len(i) == 0:
+        return float("nan"), 0
+    j = i + 1
+    differ = y[i] != y[j]
+    i, j = i[differ], j[differ]
+    good = (s[j] < s[i]) == (y[j] < y[i])
+    return float(good.mean()), len(i)
+
+
+def train_scorer(X, y, epochs=8, batch=2048, lr=2e-3, seed=0, log=print):
+    s = G.Scorer(seed)
+    t0 = time.perf_counter()
+    losses = s.train(X, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    log("trained on %d samples, %d epochs in %.1f s; loss per epoch %s"
+        % (len(y), epochs, time.perf_counter() - t0, np.array2string(losses, precision=4)))
+    return s
+
+
+def knot_batch(nq, seed):
+    tris, idx = W.torus_knot()
+    t9 = tris.reshape(-1, 9)
+    meshes = [G.Mesh(t9, vidx=idx), G.Mesh(t9, vidx=idx)]
+    P = W.knot_queries(nq, seed, collide_frac=0.3)
+    return meshes, P, None
+
+
+def uneven_batch(nq, seed):
+    tab = W.uneven_meshes()
+    meshes = [G.Mesh(t, vidx=i) for t, i, _ in tab]
+    P00, P01, P10, P11, mp = W.uneven_queries(nq, seed, [len(t) for t, _, _ in tab],
+                                              [r for _, _, r in tab])
+    return meshes, (P00, P01, P10, P11), mp
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_SCORER)
+    ap.add_argument("--knot-nq", type=int, default=4096)
+    ap.add_argument("--uneven-nq", type=int, default=16384)
+    ap.add_argument("--rate", type=float, default=0.25)
+    ap.add_argument("--cap", type=int, default=6_000_000)
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--split-seed", type=int, default=7)
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args()
+    Xs, ys, ks = [], [], []
+    for name, (meshes, P, mp), cfg in (
+            ("knot", knot_batch(a.knot_nq, W.CONFIG_KNOT_BATCH["seed"]), W.CONFIG_KNOT_BATCH),
+            ("uneven", uneven_batch(a.uneven_nq, W.CONFIG_UNEVEN["seed"]), W.CONFIG_UNEVEN)):
+        q = G.make_queries(*P, mesh_pairs=mp)
+        X, y, k, n = collect_samples(meshes, q, cfg["tol_d"], cfg["tol_t"], a.rate, a.cap)
+        m = split_mask(k, 0.8, a.split_seed)
+        print("%s: %d samples selected, %d kept, %d for training" % (name, n, len(y), m.sum()),
+              flush=True)
+        Xs.append(X[m])
+        ys.append(y[m])
+        ks.append(k[m])
+    X, y = np.concatenate(Xs), np.concatenate(ys)
+    s = train_scorer(X, y, epochs=a.epochs, seed=a.seed)
+    s.save(a.out)
+    print("saved", a.out)
+
+
+if __name__ == "__main__":
+    main()

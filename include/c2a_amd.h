@@ -39,6 +39,7 @@ typedef enum {
 
 typedef struct c2a_mesh c2a_mesh;
 typedef struct c2a_batch c2a_batch;
+typedef struct c2a_scorer c2a_scorer;   /* learned front ordering, see below */
 
 /* One CCD query: start/end poses of object 1 then object 2, each a
  * row-major 3x3 rotation followed by the translation (12 doubles).  The
@@ -67,8 +68,17 @@ typedef struct {
                                build provides 1 (0 = 1) */
   int32_t replay_log;       /* expansions logged per CA step for the next step's
                                replay (default 8192); 0 disables replay */
-  int32_t pad;
+  int32_t order;            /* C2A_ORDER_PLAIN (the reference's closer-child-first
+                               order, bit-exact) or C2A_ORDER_LEARNED */
+  const c2a_scorer *scorer; /* C2A_ORDER_LEARNED: visits the sibling with the
+                               lower score first; read at each run; never
+                               prunes a pair */
+  double sample_rate;       /* fraction of replayed pairs logged as training
+                               samples (selected by hashing their key) */
+  int64_t sample_cap;       /* rows of the sample buffer; 0 = no logging */
 } c2a_params;
+
+enum { C2A_ORDER_PLAIN = 0, C2A_ORDER_LEARNED = 1 };
 
 /* Per-query result (C2A_TimeOfContactResult, C2A.h:161-224, plus the
  * reference's per-model last_tri after the query). */
@@ -130,6 +140,16 @@ int c2a_batch_destroy(c2a_batch *b);
 /* Kernel-only device time of the last c2a_batch_run, in milliseconds,
  * measured with HIP events on the stream the kernel was launched on. */
 int c2a_batch_last_ms(c2a_batch *b, float *ms);
+/* Training samples logged by the last c2a_batch_run (params->sample_cap > 0;
+ * replay_log > 0): every CA step after the first replays the previous
+ * step's traversal, and each replayed pair whose replayed subtree holds a
+ * leaf pair is a sample -- its 32 scorer features (BF16 values as FP32), the
+ * label log(closest replayed triangle distance in that subtree + 1e-4) and
+ * a key (query << 34 | CA step << 26 | record).  Up to `cap` rows sorted by
+ * key are written; *n_out receives the number of samples selected, which
+ * may exceed sample_cap (the rows kept past the cap depend on timing). */
+int c2a_batch_samples(c2a_batch *b, float *feats, float *labels, int64_t *keys, int64_t cap,
+                      int64_t *n_out, int64_t *n_written);
 
 /* One conservative-advancement step at explicit poses (backs
  * C2A_TimeOfContactStep).  state_in: numCA, previous mint and UpboundTOC as
@@ -179,11 +199,14 @@ int c2a_motion_pose(const double *motion24, double t, double *pose12, double *qu
  * b1[64], W2[64][64], b2[64], w3[64], b3).  The feature list is frozen in
  * c2a_amd/csrc/scorer.h; the target is log(closest triangle distance in the
  * pair's subtree + 1e-4). */
-typedef struct c2a_scorer c2a_scorer;
 int c2a_scorer_create(uint64_t seed, c2a_scorer **out);   /* He-uniform weights */
 int c2a_scorer_destroy(c2a_scorer *s);
 int c2a_scorer_set_params(c2a_scorer *s, const float *params6337);
 int c2a_scorer_get_params(const c2a_scorer *s, float *params6337);
+/* Parameter file: "C2ASCR01", uint32 count (6337), FP32 parameters, all
+ * little-endian.  Loading checks the header, the count and finiteness. */
+int c2a_scorer_save(const c2a_scorer *s, const char *path);
+int c2a_scorer_load(const char *path, c2a_scorer **out);
 /* Scores of n feature rows (n x 32 FP32). */
 int c2a_scorer_infer(const c2a_scorer *s, const float *feats, int64_t n, float *out);
 /* Gradient (6337) and value of the mean squared error over n rows. */

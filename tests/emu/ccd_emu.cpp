@@ -4,6 +4,7 @@
 // GPU.  The product library never links this file.
 #include <string.h>
 
+#include <algorithm>
 #include <memory>
 #include <vector>
 
@@ -30,9 +31,54 @@ static int emu_arena_rounds = 1;   // the CPU harness exercises the arena-round 
 extern "C" void emu_set_arena_rounds(int on) { emu_arena_rounds = on; }
 extern "C" int64_t emu_round_kind(int later, int kind) { return g_kind[later][kind]; }
 
+// Learned order on the CPU: a host copy of a scorer (scalar forward pass, so
+// scores may differ from the matrix cores' in the last bits).
+static std::vector<uint16_t> g_w1t, g_w2t;
+static std::vector<float> g_b1, g_b2, g_w3;
+static ScorerDev g_scorer;
+extern "C" void emu_set_scorer(const float* params) {
+  memset(&g_scorer, 0, sizeof(g_scorer));
+  if (!params) return;
+  g_w1t.assign(NF * NH, 0);
+  g_w2t.assign(NH * NH, 0);
+  for (int k = 0; k < NF; k++)
+    for (int j = 0; j < NH; j++) g_w1t[j * NF + k] = f32_to_bf16(params[P_W1 + k * NH + j]);
+  for (int k = 0; k < NH; k++)
+    for (int j = 0; j < NH; j++) g_w2t[j * NH + k] = f32_to_bf16(params[P_W2 + k * NH + j]);
+  g_b1.assign(params + P_B1, params + P_B1 + NH);
+  g_b2.assign(params + P_B2, params + P_B2 + NH);
+  g_w3.assign(params + P_W3, params + P_W3 + NH);
+  g_scorer.w1t = g_w1t.data();
+  g_scorer.w2t = g_w2t.data();
+  g_scorer.b1 = g_b1.data();
+  g_scorer.b2 = g_b2.data();
+  g_scorer.w3 = g_w3.data();
+  g_scorer.b3 = params[P_B3];
+  g_scorer.on = 1;
+}
+
+// Training-sample logging on the CPU.
+static double g_rate = 0.0;
+static std::vector<float> g_samples;
+static std::vector<int64_t> g_keys;
+static unsigned long long g_scount = 0;
+extern "C" void emu_set_sampling(double rate, int64_t cap) {
+  g_rate = rate;
+  g_samples.assign((size_t)(cap > 0 ? cap : 0) * (NF + 1), 0.0f);
+  g_keys.assign((size_t)(cap > 0 ? cap : 0), 0);
+}
+extern "C" int64_t emu_samples(float* rows, int64_t* keys, int64_t cap) {
+  const int64_t n = std::min<int64_t>((int64_t)g_scount, (int64_t)g_keys.size());
+  for (int64_t i = 0; i < n && i < cap; i++) {
+    memcpy(rows + i * (NF + 1), &g_samples[(size_t)i * (NF + 1)], (NF + 1) * sizeof(float));
+    keys[i] = g_keys[(size_t)i];
+  }
+  return (int64_t)g_scount;
+}
+
 template <int K, int CAP>
 static int run(const KArgs& a) {
-  std::vector<unsigned char> arena(a.logcap > 0 ? arena_bytes(a.logcap) : 1);
+  std::vector<unsigned char> arena(a.logcap > 0 ? arena_bytes(a.logcap, a.feats != 0) : 1);
   using W = Wave<K, CAP>;
   std::unique_ptr<W> gp(new W());
   W& g = *gp;
@@ -43,10 +89,22 @@ static int run(const KArgs& a) {
   L.ab = a.logcap > 0 ? arena.data() : nullptr;
   g_rounds = 0;
   memset(g_kind, 0, sizeof(g_kind));
+  g_scount = 0;
   for (;;) {
     leader_run(g, L, a, 0, 1);
     if (L.state == ST_DONE) break;
+    if (L.need_score) {
+      const ArenaView av = arena_view(L.ab, a.logcap, true);
+      for (int r = 1; r < 1 + 2 * L.nlog_prev; r++)
+        av.hot[r].score = score_dev_scalar(a.scorer, av.feat + (int64_t)r * NF);
+      continue;
+    }
     g_rounds++;
+    if (round_scored(g, a)) {
+      round_lane_scored(g, L, 0, 1, a);
+      for (int h = 1; h < Shape<K>::NSLOT; h++) g.slot[h].score = score_dev_scalar(a.scorer, g.fx[h]);
+      continue;
+    }
     round_lane(g, L, 0, 1, a);
   }
   return 0;
@@ -91,6 +149,15 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
   a.mode = MODE_QUERY;
   a.logcap = p->replay_log;
   a.arena_rounds = emu_arena_rounds;
+  a.scorer = g_scorer;
+  if (!g_keys.empty() && a.logcap > 0) {
+    a.sample_thr = (uint64_t)((g_rate < 0 ? 0 : g_rate > 1 ? 1 : g_rate) * 4294967296.0);
+    a.samples = g_samples.data();
+    a.sample_keys = g_keys.data();
+    a.sample_cap = (int64_t)g_keys.size();
+    a.sample_count = &g_scount;
+  }
+  a.feats = (a.scorer.on || a.sample_cap > 0) ? 1 : 0;
   // stack capacity as the library picks it
   const bool small = b1.max_depth + b2.max_depth + 2 <= 48;
   switch (K) {

@@ -70,3 +70,80 @@ def test_kernel_logic_unequal_meshes_and_seeds(emu):
     m1, m2 = O.OModel(a), O.OModel(b)
     ref = O.batch_toc(m1, m2, *P, 1e-4, 1e-4, seeds=seeds, with_contacts=True, nthreads=4)
     same(ref, got)
+
+
+@pytest.fixture()
+def emu_learned(emu):
+    emu.emu_set_scorer.argtypes = [C.c_void_p]
+    emu.emu_set_sampling.argtypes = [C.c_double, C.c_int64]
+    emu.emu_samples.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    emu.emu_samples.restype = C.c_int64
+    yield emu
+    emu.emu_set_scorer(None)
+    emu.emu_set_sampling(0.0, 0)
+
+
+def emu_samples(emu, cap):
+    rows = np.zeros((cap, G.NF + 1), np.float32)
+    keys = np.zeros(cap, np.int64)
+    n = emu.emu_samples(rows.ctypes.data, keys.ctypes.data, cap)
+    w = min(n, cap)
+    o = np.argsort(keys[:w], kind="stable")
+    return rows[:w][o], keys[:w][o], n
+
+
+def _knot_case(nq=16, seed=900):
+    tris, _ = W.torus_knot(n_seg=128, n_ring=12)
+    t9 = tris.reshape(-1, 9)
+    return t9, W.knot_queries(nq, seed, collide_frac=0.4)
+
+
+def test_sampling_keeps_results_and_selects_by_hash(emu_learned):
+    """Logging training samples changes no result; the hash selection at a
+    lower rate picks a subset of the same rows."""
+    emu = emu_learned
+    t9, P = _knot_case()
+    params = G.default_params(tol_d=1e-3, tol_t=1e-4)
+    base = run_emu(emu, t9, t9, P, params, 5)
+    emu.emu_set_sampling(1.0, 400000)
+    full = run_emu(emu, t9, t9, P, params, 5)
+    rows1, keys1, n1 = emu_samples(emu, 400000)
+    emu.emu_set_sampling(0.25, 400000)
+    run_emu(emu, t9, t9, P, params, 5)
+    rows2, keys2, n2 = emu_samples(emu, 400000)
+    for f in base.dtype.names:
+        assert np.array_equal(base[f], full[f]), f
+    assert n1 == len(keys1) > 1000 and len(np.unique(keys1)) == n1
+    assert 0.15 * n1 < n2 < 0.35 * n1
+    pos = np.searchsorted(keys1, keys2)
+    assert np.array_equal(keys1[pos], keys2)
+    assert np.array_equal(rows1[pos], rows2)
+    # every sample is a replayed pair below the root of a step after the first
+    step = (keys1 >> 26) & 0xff
+    rec = keys1 & 0x3ffffff
+    assert (rec > 0).all() and (step >= 1).all()
+    assert np.isfinite(rows1).all()
+    # leaf-flag feature: (leaf1 + 2 leaf2) / 3; a leaf pair's label is its own
+    # triangle distance, so it is never below log(1e-4)
+    lf = np.round(rows1[:, 31] * 3).astype(int)
+    assert set(np.unique(lf)) <= {0, 1, 2, 3} and (lf == 3).any()
+    assert (rows1[:, G.NF] >= np.float32(np.log(1e-4)) - 1e-6).all()
+
+
+def test_learned_order_is_conservative_on_cpu(emu_learned):
+    """A (random) scorer only reorders siblings: every query still ends with
+    the plain order's collision flag, a TOC no later than the plain TOC plus
+    tol_t, and a different traversal."""
+    emu = emu_learned
+    t9, P = _knot_case(24, 901)
+    params = G.default_params(tol_d=1e-3, tol_t=1e-4)
+    plain = run_emu(emu, t9, t9, P, params, 5)
+    rng = np.random.default_rng(5)
+    p = (rng.standard_normal(6337) * 0.2).astype(np.float32)
+    emu.emu_set_scorer(p.ctypes.data)
+    learned = run_emu(emu, t9, t9, P, params, 5)
+    assert np.array_equal(plain["collisionfree"], learned["collisionfree"])
+    assert (learned["toc"] <= plain["toc"] + 1e-4).all()
+    hit = plain["collisionfree"] == 0
+    assert (learned["distance"][hit] <= 1e-3).all()
+    assert not np.array_equal(plain["bv_tests"], learned["bv_tests"])
