@@ -8,8 +8,14 @@ U[-2,2]^3, uniform random rotation; 30% of end poses pulled toward
 collision), tolerance_d = 1e-3, tolerance_t = 1e-4.  Inputs are uploaded
 once; the timed region covers the kernel launches only.
 
-Prints one JSON line (see README of the driver contract).  The CPU baseline is
-the FP64 oracle timed on this host's cores over the same workload.
+Prints one JSON line.  `value` is the plain (reference-order, bit-exact)
+throughput; the `learned` object times the same batch with the shipped
+learned front ordering (c2a_amd/scorer_knot_uneven.c2as) and reports its BV
+tests, TOC error against the plain order and a conservativeness check: the
+queries where the orders disagree most are re-solved by the FP64 oracle at
+tol_d 1e-6, and no learned TOC may lie after that near-exact contact time.
+The CPU baseline is the FP64 oracle timed on this host's cores over a
+bounded sample of the same workload.
 """
 import argparse
 import json
@@ -38,6 +44,9 @@ def parse():
     ap.add_argument("--workload", choices=["knot", "uneven", "grazing"], default="knot",
                     help="knot: the headline batch; uneven / grazing: the other configurations")
     ap.add_argument("--spec-depth", type=int, default=0)
+    ap.add_argument("--order", choices=["both", "plain", "learned"], default="both",
+                    help="time the plain order, the learned order, or both (value = plain)")
+    ap.add_argument("--scorer", default="", help="scorer file (default: the shipped one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 256 MiB device copy (to scale HBM counters)")
@@ -80,6 +89,52 @@ def cpu_baseline(W, nq_sample):
     return {"value": nq_sample / dt, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": "first %d queries of the same workload, %d threads, %.2f s wall"
                       % (nq_sample, threads, dt)}
+
+
+def tight_check(wl, toc_learned, sel):
+    """Near-exact contact times (FP64 oracle at tol_d 1e-6, tol_t 1e-9, up to
+    5000 steps) for the queries `sel`; returns (checked, late) where late
+    counts learned TOCs after the near-exact time (non-conservative)."""
+    import numpy as np
+
+    from oracle import oracle as O
+    mp = wl["pairs"]
+    checked = late = 0
+    for a, b in sorted(set(map(tuple, mp[sel]))):
+        s = sel[(mp[sel, 0] == a) & (mp[sel, 1] == b)]
+        m1 = O.OModel(wl["tris"][a])
+        m2 = O.OModel(wl["tris"][b])
+        ref = O.batch_toc(m1, m2, *[p[s] for p in wl["P"]], 1e-6, 1e-9,
+                          seeds=np.zeros((len(s), 2), np.int32), max_iter=5000,
+                          nthreads=min(16, os.cpu_count() or 1))
+        ok = ((ref["flags"] & 1) == 0) & (ref["collisionfree"] == 0)
+        checked += int(ok.sum())
+        late += int((toc_learned[s][ok] > ref["toc"][ok]).sum())
+    return checked, late
+
+
+def time_batch(batch, steps, warmup):
+    """Warm-up launches, then `steps` timed launches; (results, wall ms per
+    launch, mean kernel ms)."""
+    import numpy as np
+    import torch
+    for _ in range(warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    ref = batch.fetch() if warmup > 0 else None
+    kernel_ms = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+        kernel_ms.append(batch.last_ms())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    res = batch.fetch()
+    for f in ("toc", "distance", "numCA", "bv_tests", "tri_tests"):
+        if ref is not None and not np.array_equal(ref[f], res[f]):
+            raise SystemExit("bench.py: results changed between launches (%s)" % f)
+    return res, wall * 1000.0 / steps, float(np.mean(kernel_ms))
 
 
 def load_workload(name, nq):
@@ -145,33 +200,45 @@ def main():
         meshes = [G.Mesh(t9, vidx=idx), G.Mesh(t9, vidx=idx)]
     else:
         meshes = [G.Mesh(t, vidx=i) for t, i in zip(wl["tris"], wl["vidx"])]
-    params = G.default_params(tol_d=wl["tol_d"], tol_t=wl["tol_t"], with_contacts=0,
-                              spec_depth=args.spec_depth)
-    batch = G.Batch(meshes, G.make_queries(*wl["P"], mesh_pairs=wl["pairs"]), params)
-
-    for _ in range(args.warmup):
-        batch.run()
-    torch.cuda.synchronize()
-    ref = batch.fetch() if args.warmup > 0 else None
-    kernel_ms = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.run()
-        kernel_ms.append(batch.last_ms())
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    res = batch.fetch()
-    for f in ("toc", "distance", "numCA", "bv_tests", "tri_tests"):
-        if ref is not None and not np.array_equal(ref[f], res[f]):
-            raise SystemExit("bench.py: results changed between launches (%s)" % f)
-
-    ms_per_step = wall * 1000.0 / args.steps
+    queries = G.make_queries(*wl["P"], mesh_pairs=wl["pairs"])
+    res = learned = None
+    if args.order in ("both", "plain"):
+        params = G.default_params(tol_d=wl["tol_d"], tol_t=wl["tol_t"], with_contacts=0,
+                                  spec_depth=args.spec_depth)
+        res, ms_per_step, kms = time_batch(G.Batch(meshes, queries, params), args.steps,
+                                           args.warmup)
+    if args.order in ("both", "learned"):
+        from c2a_amd import learn as Lm
+        scorer = G.Scorer(path=args.scorer or Lm.DEFAULT_SCORER)
+        lp = G.default_params(tol_d=wl["tol_d"], tol_t=wl["tol_t"], with_contacts=0,
+                              order=G.ORDER_LEARNED, scorer=scorer)
+        lres, lms_step, lkms = time_batch(G.Batch(meshes, queries, lp, scorer=scorer),
+                                          args.steps, args.warmup)
+        learned = {"value": nq / (lms_step / 1000.0), "ms_per_step": lms_step, "kernel_ms": lkms,
+                   "bv_tests": int(lres["bv_tests"].sum()),
+                   "scorer": os.path.basename(args.scorer or Lm.DEFAULT_SCORER)}
+        if res is not None:
+            dt = lres["toc"] - res["toc"]
+            hit = np.nonzero(lres["collisionfree"] == 0)[0]
+            worst = hit[np.argsort(-np.abs(dt[hit]))][:64]
+            checked, late = tight_check(wl, lres["toc"], worst)
+            learned.update({
+                "speedup_vs_plain": ms_per_step / lms_step,
+                "bv_ratio_vs_plain": float(res["bv_tests"].sum() / max(1, lres["bv_tests"].sum())),
+                "toc_err_max": float(np.abs(dt).max()),
+                "toc_err_mean": float(np.abs(dt).mean()),
+                "toc_within_1e-4": float((np.abs(dt) <= 1e-4).mean()),
+                "collision_flag_mismatches": int((lres["collisionfree"] != res["collisionfree"]).sum()),
+                "conservativeness": {"checked": checked, "non_conservative": late,
+                                     "sample": "the %d colliding queries with the largest "
+                                               "learned-vs-plain TOC difference, against the "
+                                               "FP64 oracle at tol_d 1e-6" % len(worst)}})
+        if res is None:
+            res, ms_per_step, kms = lres, lms_step, lkms
     value = nq / (ms_per_step / 1000.0)
     algo_bytes = (BYTES_PER_BV_TEST * int(res["bv_tests"].sum()) +
                   BYTES_PER_TRI_TEST * int(res["tri_tests"].sum()) +
                   BYTES_PER_CA_STEP * int(res["numCA"].sum()))
-    kms = float(np.mean(kernel_ms))
     achieved = algo_bytes / (kms / 1000.0) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "ccd_kernel_knot4096_traffic.json")
@@ -195,7 +262,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": wl["desc"],
-            "order": "plain (reference DFS order, bit-exact)",
+            "order": ("plain (reference DFS order, bit-exact)" if args.order != "learned"
+                      else "learned (scorer-ordered siblings)"),
             "nq": nq,
             "spec_depth": args.spec_depth or 5,
             "bv_tests": int(res["bv_tests"].sum()),
@@ -214,6 +282,8 @@ def main():
             "traffic": traffic,
         },
     }
+    if learned is not None and args.order == "both":
+        out["learned"] = learned
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, wl["sample"])
     print(json.dumps(out), flush=True)

@@ -739,7 +739,8 @@ C2A_HD void write_samples(const LeaderT& L, const KArgs& a, const ArenaView& av,
     const uint64_t m = wave_ballot(take);
     if (!m) continue;
     long long start = 0;
-    if (lane == 0) start = (long long)global_add_u64(a.sample_count, (unsigned long long)__builtin_popcountll(m));
+    if ((lane & 63) == 0)   // one claim per wavefront
+      start = (long long)global_add_u64(a.sample_count, (unsigned long long)__builtin_popcountll(m));
     start = lane0_bcast64(start);
     const long long slot = start + lanes_below(m, lane);
     if (take && slot < a.sample_cap) {

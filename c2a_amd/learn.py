@@ -54,9 +54,10 @@ def query_split(nq, train_frac, seed):
 
 
 def sibling_accuracy(keys, labels, scores):
-    """Fraction of sibling pairs (records 2e+1, 2e+2 of one step) whose order
-    by score puts the child with the closer replayed leaf first, and the same
-    for the plain order's RSS distance (feature 18 = log d) if scores is None."""
+    """Fraction of sibling pairs (records 2e+1, 2e+2 of one step, both
+    sampled, with different labels) that `scores` order the way the labels
+    do (lower first), and the number of such pairs.  Passing feature 18
+    (log RSS distance) as the scores measures the plain order."""
     o = np.argsort(keys, kind="stable")
     k, y, s = keys[o], labels[o], scores[o]
     first = (k & 1) == 1            # record 2e+1
@@ -97,33 +98,73 @@ def uneven_batch(nq, seed):
     return meshes, (P00, P01, P10, P11), mp
 
 
+def compare(plain, learned, mask=None):
+    """Work and TOC agreement of a learned-order run against the plain run."""
+    if mask is not None:
+        plain, learned = plain[mask], learned[mask]
+    dt = learned["toc"] - plain["toc"]
+    return {
+        "queries": int(len(plain)),
+        "bv_tests_plain": int(plain["bv_tests"].sum()),
+        "bv_tests_learned": int(learned["bv_tests"].sum()),
+        "bv_ratio": float(plain["bv_tests"].sum() / max(1, learned["bv_tests"].sum())),
+        "toc_err_max": float(np.abs(dt).max()) if len(dt) else 0.0,
+        "toc_err_mean": float(np.abs(dt).mean()) if len(dt) else 0.0,
+        "toc_within_1e-4": float((np.abs(dt) <= 1e-4).mean()) if len(dt) else 1.0,
+        "collision_flag_mismatches": int((plain["collisionfree"] != learned["collisionfree"]).sum()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_SCORER)
     ap.add_argument("--knot-nq", type=int, default=4096)
     ap.add_argument("--uneven-nq", type=int, default=16384)
-    ap.add_argument("--rate", type=float, default=0.25)
-    ap.add_argument("--cap", type=int, default=6_000_000)
-    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--rate", type=float, default=0.1)
+    ap.add_argument("--cap", type=int, default=12_000_000)
+    ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--split-seed", type=int, default=7)
     ap.add_argument("--seed", type=int, default=1)
     a = ap.parse_args()
-    Xs, ys, ks = [], [], []
-    for name, (meshes, P, mp), cfg in (
-            ("knot", knot_batch(a.knot_nq, W.CONFIG_KNOT_BATCH["seed"]), W.CONFIG_KNOT_BATCH),
-            ("uneven", uneven_batch(a.uneven_nq, W.CONFIG_UNEVEN["seed"]), W.CONFIG_UNEVEN)):
+    Xs, ys, runs = [], [], {}
+    for name, (meshes, P, mp), cfg, nq in (
+            ("knot", knot_batch(a.knot_nq, W.CONFIG_KNOT_BATCH["seed"]), W.CONFIG_KNOT_BATCH,
+             a.knot_nq),
+            ("uneven", uneven_batch(a.uneven_nq, W.CONFIG_UNEVEN["seed"]), W.CONFIG_UNEVEN,
+             a.uneven_nq)):
         q = G.make_queries(*P, mesh_pairs=mp)
         X, y, k, n = collect_samples(meshes, q, cfg["tol_d"], cfg["tol_t"], a.rate, a.cap)
+        if n > a.cap:
+            raise SystemExit("%s: %d samples selected, above the cap %d (lower --rate)"
+                             % (name, n, a.cap))
         m = split_mask(k, 0.8, a.split_seed)
-        print("%s: %d samples selected, %d kept, %d for training" % (name, n, len(y), m.sum()),
-              flush=True)
+        acc, npair = sibling_accuracy(k[~m], y[~m], X[~m, 18])
+        print("%s: %d samples, %d for training; held-out sibling pairs %d, plain-order "
+              "accuracy %.4f" % (name, n, m.sum(), npair, acc), flush=True)
         Xs.append(X[m])
         ys.append(y[m])
-        ks.append(k[m])
+        runs[name] = (meshes, q, cfg, nq, X[~m], y[~m], k[~m])
     X, y = np.concatenate(Xs), np.concatenate(ys)
     s = train_scorer(X, y, epochs=a.epochs, seed=a.seed)
     s.save(a.out)
     print("saved", a.out)
+    for name, (meshes, q, cfg, nq, Xv, yv, kv) in runs.items():
+        sc = s.infer(Xv)
+        acc, npair = sibling_accuracy(kv, yv, sc)
+        print("%s held-out: mse %.4f, learned sibling accuracy %.4f over %d pairs"
+              % (name, float(np.mean((sc - yv) ** 2)), acc, npair))
+        res = {}
+        for order in ("plain", "learned"):
+            kw = dict(order=G.ORDER_LEARNED, scorer=s) if order == "learned" else {}
+            b = G.Batch(meshes, q, G.default_params(tol_d=cfg["tol_d"], tol_t=cfg["tol_t"],
+                                                    with_contacts=0, **kw))
+            b.run()
+            b.run()
+            res[order] = (b.fetch(), b.last_ms())
+        held = ~query_split(nq, 0.8, a.split_seed)
+        c = compare(res["plain"][0], res["learned"][0], held)
+        c.update(plain_ms=res["plain"][1], learned_ms=res["learned"][1])
+        print("%s held-out queries: %s" % (name, c), flush=True)
 
 
 if __name__ == "__main__":
