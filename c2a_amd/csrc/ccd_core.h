@@ -195,8 +195,6 @@ struct Slot {            // speculative pair values, heap-indexed below X
   double dTri, mleaf;    // leaf pairs: triangle distance and leaf admissible time
   double p[3], q[3];     // leaf pairs: closest points in model-1 frame
   int32_t t1, t2;
-  float score;           // learned order: the scorer's output for the pair
-  int32_t spad;
 };
 
 template <int K>
@@ -209,7 +207,6 @@ template <int K, int CAP>
 struct Wave {
   Entry stack[CAP];
   Slot slot[Shape<K>::NSLOT];
-  uint16_t fx[64][NF];       // learned order: BF16 features of the round's slots
   Entry X;                   // root of the pending round
   double Rrel[9], Trel[3];   // model 2 in model 1's frame (leaf tests)
   double r1[9], tt1[3];      // current pose of object 1 (world directions)
@@ -401,11 +398,10 @@ C2A_HD void pair_feature_row(uint16_t* out, node_ptr N1, node_ptr N2, int b1, in
 }
 
 // Heap slot h of the depth-K subtree below X: the pair reached by the path
-// bits of h+1, its transform chained from X, and its values.  fx (optional):
-// where the pair's scorer features go (zeros for a slot below a leaf pair).
+// bits of h+1, its transform chained from X, and its values.
 template <int K, int CAP>
 C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
-                              bool contact, Slot& s, uint16_t* fx = nullptr, double up = 0.0) {
+                              bool contact, Slot& s) {
   node_ptr N1 = nodes_of(M1);
   node_ptr N2 = nodes_of(M2);
   const int lvl = 31 - clz32((unsigned)(h + 1));
@@ -417,15 +413,12 @@ C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const M
     const int bit = ((h + 1) >> (lvl - 1 - lv)) & 1;
     if (!descend(N1, N2, b1, b2, bit, R, T)) {
       s.valid = 0;
-      if (fx)
-        for (int k = 0; k < NF; k++) fx[k] = 0;
       return;
     }
   }
-  double d, mt, mb[2] = {0.0, 0.0};
+  double d, mt;
   C2A_CHECK(b1 >= 0 && b2 >= 0 && b1 < 2 * M1.ntri - 1 && b2 < 2 * M2.ntri - 1);
-  const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], contact, d, mt, mb);
-  if (fx) pair_feature_row(fx, N1, N2, b1, b2, R, T, d, mt, mb, g.mo[0], g.mo[1], up, fl);
+  const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], contact, d, mt);
   for (int i = 0; i < 9; i++) s.R[i] = R[i];
   for (int i = 0; i < 3; i++) s.T[i] = T[i];
   s.d = d;
@@ -440,8 +433,8 @@ C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const M
 
 template <int K, int CAP>
 C2A_HD void compute_slot(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
-                         bool contact, uint16_t* fx = nullptr, double up = 0.0) {
-  compute_slot_into(g, h, M1, M2, contact, g.slot[h], fx, up);
+                         bool contact) {
+  compute_slot_into(g, h, M1, M2, contact, g.slot[h]);
 }
 
 // Heap slot h of the round written as arena record 2 * rpb + h, so that the
@@ -477,13 +470,6 @@ C2A_HD void compute_slot_arena(Wave<K, CAP>& g, int h, const MeshDev& M1, const 
   gp(av.hot)[rec] = hr;
 }
 
-// Whether the pending round's slots are ordered by the scorer: CA-step
-// rounds below an internal pair (the scorer covers up to 64 slots).
-template <int K, int CAP>
-C2A_HD bool round_scored(const Wave<K, CAP>& g, const KArgs& a) {
-  return Shape<K>::NSLOT <= 64 && a.scorer.on && !g.contact && !g.rarena && (g.X.flags & 3) != 3;
-}
-
 // One lane's share of a round.
 template <int K, int CAP>
 C2A_HD void round_lane(Wave<K, CAP>& g, const Leader& L, int lane, int nl, const KArgs& a) {
@@ -508,17 +494,6 @@ C2A_HD void round_lane(Wave<K, CAP>& g, const Leader& L, int lane, int nl, const
     return;
   }
   for (int h = lane + 1; h < Shape<K>::NSLOT; h += nl) compute_slot(g, h, M1, M2, contact);
-}
-
-// A round whose slots the scorer orders (round_scored): the slots and their
-// features; the caller then scores them.
-template <int K, int CAP>
-C2A_HD void round_lane_scored(Wave<K, CAP>& g, const Leader& L, int lane, int nl,
-                              const KArgs& a) {
-  const MeshDev M1 = a.meshes[g.mesh[0]];
-  const MeshDev M2 = a.meshes[g.mesh[1]];
-  for (int h = lane + 1; h < Shape<K>::NSLOT; h += nl)
-    compute_slot(g, h, M1, M2, false, &g.fx[h][0], L.Up);
 }
 
 template <int K, int CAP>
@@ -1249,7 +1224,7 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
       const Cur cA = cur_from_hot(g.win[ra - L.wbase], ra, nnA, clvl);
       const Cur cC = cur_from_hot(g.win[ra + 1 - L.wbase], ra + 1, nnC, clvl);
       // closer child visited first (C2A.cpp:1281), or the lower score under
-      // the learned order; the other one waits
+      // the learned order (replayed pairs are scored); the other one waits
       const bool c_first = learned ? unif(g.win[ra + 1 - L.wbase].score) <
                                          unif(g.win[ra - L.wbase].score)
                                    : cC.d < cA.d;
@@ -1263,9 +1238,10 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
       continue;
     }
     const int sa = 2 * s + 1, sc = 2 * s + 2;
+    // pairs first reached in this step keep the closer-child-first order:
+    // scoring a round's slots costs more than the better order saves
     const double dA = unid(g.slot[sa].d), dC = unid(g.slot[sc].d);
-    const bool c_first =
-        learned ? unif(g.slot[sc].score) < unif(g.slot[sa].score) : dC < dA;
+    const bool c_first = dC < dA;
     const int sf = c_first ? sc : sa, ss = c_first ? sa : sc;
     const int nf = c_first ? nnC : nnA, ns = c_first ? nnA : nnC;
     const Slot& F = g.slot[sf];

@@ -50,14 +50,6 @@ __global__ void __launch_bounds__(64 * NW) ccd_kernel(KArgs a) {
         score_replay(g, L, a, lane);
         continue;
       }
-      if (round_scored(g, a)) {
-        round_lane_scored(g, L, lane, 64 * NW, a);
-        lane_sync();
-        const float y = score64(a.scorer, &g.fx[0][0], lane);
-        if (lane >= 1 && lane < Shape<K>::NSLOT) g.slot[lane].score = y;
-        lane_sync();
-        continue;
-      }
     }
     round_lane(g, L, lane, 64 * NW, a);
     if (g.rarena) mem_sync();   // the walk reads the new records through the LDS window

@@ -100,11 +100,6 @@ static int run(const KArgs& a) {
       continue;
     }
     g_rounds++;
-    if (round_scored(g, a)) {
-      round_lane_scored(g, L, 0, 1, a);
-      for (int h = 1; h < Shape<K>::NSLOT; h++) g.slot[h].score = score_dev_scalar(a.scorer, g.fx[h]);
-      continue;
-    }
     round_lane(g, L, 0, 1, a);
   }
   return 0;
