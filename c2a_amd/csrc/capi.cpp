@@ -156,6 +156,19 @@ int pick_shape(const c2a_params* p, int& K, int& NW) {
   return C2A_OK;
 }
 
+// Replay log capacity from the largest mesh: a CA step whose expansions
+// overflow the log walks the rest sequentially at the next step.  The
+// heaviest steps measured expand about ntri / 20 pairs (11.5k at 262,144
+// triangles), so the capacity is the next power of two of ntri / 8, at
+// least 8192 (arena ≈ 1 KB per expansion per resident workgroup).
+int auto_replay_log(const std::vector<MeshDev>& tab) {
+  int64_t big = 0;
+  for (const MeshDev& m : tab) big = std::max<int64_t>(big, m.ntri);
+  int cap = 8192;
+  while (cap < big / 8 && cap < (1 << 20)) cap <<= 1;
+  return cap;
+}
+
 // Resident workgroups: the kernel is persistent, so launching more than fit
 // at once only adds workgroups that find the queue empty.
 int grid_blocks(int K, int CAP, int NW, int64_t nq, bool learned = false) {
@@ -252,7 +265,7 @@ void c2a_params_default(c2a_params* p) {
   p->contact_offset = 0.001;
   p->spec_depth = 0;
   p->waves_per_query = 0;
-  p->replay_log = 8192;
+  p->replay_log = -1;
   p->order = C2A_ORDER_PLAIN;
   p->scorer = nullptr;
   p->sample_rate = 0.0;
@@ -359,15 +372,15 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     c2a_params def;
     c2a_params_default(&def);
     const c2a_params* pp = params ? params : &def;
-    if (pp->replay_log < 0 || pp->replay_log > (1 << 24)) {
+    if (pp->replay_log < -1 || pp->replay_log > (1 << 24)) {
       delete b;
-      return fail(C2A_ERR_BAD_ARG, "replay_log must be 0..2^24");
+      return fail(C2A_ERR_BAD_ARG, "replay_log must be -1 (auto) or 0..2^24");
     }
     if (pp->sample_cap < 0 || (pp->sample_cap > 0 && pp->replay_log == 0)) {
       delete b;
-      return fail(C2A_ERR_BAD_ARG, "sample_cap must be >= 0 and needs replay_log > 0");
+      return fail(C2A_ERR_BAD_ARG, "sample_cap must be >= 0 and needs replay_log != 0");
     }
-    b->logcap = pp->replay_log;
+    b->logcap = pp->replay_log >= 0 ? pp->replay_log : auto_replay_log(b->mtab);
     b->scorer = pp->order == C2A_ORDER_LEARNED ? pp->scorer : nullptr;
     b->sample_cap = pp->sample_cap;
   }

@@ -67,7 +67,9 @@ typedef struct {
   int32_t waves_per_query;  /* wavefronts cooperating on one query; this
                                build provides 1 (0 = 1) */
   int32_t replay_log;       /* expansions logged per CA step for the next step's
-                               replay (default 8192); 0 disables replay */
+                               replay; -1 (default) = from the largest mesh
+                               (8192, or ntri / 8 rounded up to a power of
+                               two); 0 disables replay */
   int32_t order;            /* C2A_ORDER_PLAIN (the reference's closer-child-first
                                order, bit-exact) or C2A_ORDER_LEARNED */
   const c2a_scorer *scorer; /* C2A_ORDER_LEARNED: visits the sibling with the

@@ -143,6 +143,10 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
   a.with_contacts = p->with_contacts;
   a.mode = MODE_QUERY;
   a.logcap = p->replay_log;
+  if (a.logcap < 0) {   // automatic capacity, as the library picks it
+    a.logcap = 8192;
+    while (a.logcap < std::max(n1, n2) / 8 && a.logcap < (1 << 20)) a.logcap <<= 1;
+  }
   a.arena_rounds = emu_arena_rounds;
   a.scorer = g_scorer;
   if (!g_keys.empty() && a.logcap > 0) {
