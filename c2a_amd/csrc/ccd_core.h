@@ -212,6 +212,7 @@ struct Wave {
   double r1[9], tt1[3];      // current pose of object 1 (world directions)
   Motion mo[2];
   int32_t mesh[2];
+  MeshDev md[2];             // the two meshes' table entries (set at fetch)
   int32_t contact;           // 1: contact-enumeration traversal (no motion bounds)
   int32_t gen;
   HotRec win[WIN];           // replayed records [L.wbase, L.wbase + WIN)
@@ -221,10 +222,28 @@ struct Wave {
   int32_t nleafq;            // replay: leaf records queued for the triangle pass
   int64_t ticket;            // work-queue ticket shared by the wavefronts
   int32_t rarena;            // the pending round writes its subtree to the arena
+  int32_t rdepth;            // levels of the slot heap the current round filled
   int32_t rpb;               // ... as pairs [rpb, rpb + 2^K - 1)
   int32_t lh[CAP + 2];       // replay: expansions per level, then fill cursors
   int32_t ls[CAP + 3];       // replay: first sorted expansion of each level
 };
+
+// A mesh-table entry held in LDS, made wavefront-uniform so that node and
+// triangle addresses use scalar base registers.
+template <int K, int CAP>
+C2A_HD MeshDev mesh_of(const Wave<K, CAP>& g, int k) {
+  MeshDev m = g.md[k];
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint64_t pn = (uint64_t)m.nodes, pt = (uint64_t)m.tris;
+  m.nodes = (const GpuNode*)(((uint64_t)(uint32_t)uni((int32_t)(pn >> 32)) << 32) |
+                             (uint32_t)uni((int32_t)(uint32_t)pn));
+  m.tris = (const double*)(((uint64_t)(uint32_t)uni((int32_t)(pt >> 32)) << 32) |
+                           (uint32_t)uni((int32_t)(uint32_t)pt));
+  m.ntri = uni(m.ntri);
+  m.depth = uni(m.depth);
+#endif
+  return m;
+}
 
 enum : int {
   ST_FETCH = 0, ST_STEP = 1, ST_WALK = 2, ST_CONTACT_WALK = 3, ST_DONE = 4,
@@ -473,8 +492,8 @@ C2A_HD void compute_slot_arena(Wave<K, CAP>& g, int h, const MeshDev& M1, const 
 // One lane's share of a round.
 template <int K, int CAP>
 C2A_HD void round_lane(Wave<K, CAP>& g, const Leader& L, int lane, int nl, const KArgs& a) {
-  const MeshDev M1 = a.meshes[g.mesh[0]];
-  const MeshDev M2 = a.meshes[g.mesh[1]];
+  const MeshDev M1 = mesh_of(g, 0);
+  const MeshDev M2 = mesh_of(g, 1);
   const bool contact = g.contact != 0;
   if ((g.X.flags & 3) == 3) {
     if (lane == 0) {
@@ -493,7 +512,8 @@ C2A_HD void round_lane(Wave<K, CAP>& g, const Leader& L, int lane, int nl, const
     for (int h = lane + 1; h < Shape<K>::NSLOT; h += nl) compute_slot_arena(g, h, M1, M2, av);
     return;
   }
-  for (int h = lane + 1; h < Shape<K>::NSLOT; h += nl) compute_slot(g, h, M1, M2, contact);
+  const int nslot = (2 << g.rdepth) - 1;   // levels 0..rdepth of the heap
+  for (int h = lane + 1; h < nslot; h += nl) compute_slot(g, h, M1, M2, contact);
 }
 
 template <int K, int CAP>
@@ -541,6 +561,10 @@ C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, 
       s0.flags = flags & 3;
       s0.valid = 1;
       g.gen = gen;
+      // a replayed pair that must now be expanded (a miss) usually needs
+      // only a level or two: a shallow first round has a shorter chain of
+      // dependent node loads
+      g.rdepth = ((flags & 8) && a.miss_depth > 0 && a.miss_depth < K) ? a.miss_depth : K;
     }
   }
   if (to_arena) {
@@ -741,8 +765,8 @@ C2A_HD void replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl)
   HotRec C2A_GLOBAL* hot = gp(av.hot);
   const int n = L.nlog_prev;
   const int nrec = 1 + 2 * n;
-  const MeshDev M1 = a.meshes[g.mesh[0]];
-  const MeshDev M2 = a.meshes[g.mesh[1]];
+  const MeshDev M1 = mesh_of(g, 0);
+  const MeshDev M2 = mesh_of(g, 1);
   node_ptr N1 = nodes_of(M1);
   node_ptr N2 = nodes_of(M2);
   mem_sync();   // the previous walk's log stores
@@ -1188,7 +1212,7 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
     }
     const bool from_arena = (flags & 8) && c.cp >= 0;
     const int lvl = cached ? 31 - clz32((unsigned)(s + 1)) : K;
-    if (!from_arena && (!cached || lvl >= K)) {
+    if (!from_arena && (!cached || lvl >= uni(g.rdepth))) {
       C2A_EMU_COUNT((flags & 8) ? 1 : (cached ? 2 : 3), L);
       push_back(g, L, c, where, csl, lane, nl);
       request_round(g, L, a, lane, nl);
@@ -1274,8 +1298,8 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
 template <int K, int CAP>
 C2A_HD void end_walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
   if (L.pq_pending) {
-    const MeshDev M1 = a.meshes[g.mesh[0]];
-    const MeshDev M2 = a.meshes[g.mesh[1]];
+    const MeshDev M1 = mesh_of(g, 0);
+    const MeshDev M2 = mesh_of(g, 1);
     double ta[9], tb[9], Rrel[9], Trel[3];
     gdouble_ptr g1 = tris_of(M1) + 9 * (int64_t)L.last1;
     gdouble_ptr g2 = tris_of(M2) + 9 * (int64_t)L.last2;
@@ -1293,8 +1317,8 @@ C2A_HD void end_walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int n
 
 template <int K, int CAP>
 C2A_HD void begin_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
-  const MeshDev M1 = a.meshes[g.mesh[0]];
-  const MeshDev M2 = a.meshes[g.mesh[1]];
+  const MeshDev M1 = mesh_of(g, 0);
+  const MeshDev M2 = mesh_of(g, 1);
   double R1[9], T1[3], R2[9], T2[3];
   if (L.numCA == 0) {
     for (int i = 0; i < 9; i++) { R1[i] = g.mo[0].R0[i]; R2[i] = g.mo[1].R0[i]; }
@@ -1357,8 +1381,8 @@ C2A_HD void finish_query(Wave<K, CAP>& g, Leader& L, const KArgs& a, bool collis
     r->num_contact = 0;
   }
   if (!collisionfree && a.with_contacts) {
-    const MeshDev M1 = a.meshes[g.mesh[0]];
-    const MeshDev M2 = a.meshes[g.mesh[1]];
+    const MeshDev M1 = mesh_of(g, 0);
+    const MeshDev M2 = mesh_of(g, 1);
     double R1[9], T1[3], R2[9], T2[3];
     motion_integrate(g.mo[0], L.toc, R1, T1, nullptr);
     motion_integrate(g.mo[1], L.toc, R2, T2, nullptr);
@@ -1471,6 +1495,8 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
       g.mo[1] = q.mo[1];
       g.mesh[0] = q.mesh[0];
       g.mesh[1] = q.mesh[1];
+      g.md[0] = a.meshes[q.mesh[0]];
+      g.md[1] = a.meshes[q.mesh[1]];
       g.contact = 0;
     }
     lane_sync();
@@ -1514,6 +1540,8 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     if (lane == 0) {
       g.mesh[0] = c.mesh[0];
       g.mesh[1] = c.mesh[1];
+      g.md[0] = a.meshes[c.mesh[0]];
+      g.md[1] = a.meshes[c.mesh[1]];
       g.contact = 1;
     }
     lane_sync();
@@ -1535,6 +1563,8 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     g.mo[1] = q.mo[1];
     g.mesh[0] = q.mesh[0];
     g.mesh[1] = q.mesh[1];
+    g.md[0] = a.meshes[q.mesh[0]];
+    g.md[1] = a.meshes[q.mesh[1]];
   }
   lane_sync();
   L.seed1 = q.seed[0];

@@ -168,7 +168,7 @@ struct KArgs {
   int32_t arena_rounds;    // speculative rounds write their subtree to the arena
   unsigned char* arena;    // one arena_bytes(logcap, feats) slice per workgroup
   int32_t feats;           // the arena holds features (learned order or samples)
-  int32_t pad2;
+  int32_t miss_depth;      // depth of the first round below a miss (0 = K)
   ScorerDev scorer;        // learned child order when scorer.on
   // training samples of replayed pairs: a pair is logged when the top 32
   // bits of mix64(key) are below sample_thr (2^32 = every pair)

@@ -1,15 +1,18 @@
 """Learned front ordering: training data from the exact GPU traversal, the
 scorer's training on the GPU, and its evaluation.
 
-The data come from the plain (reference-order) traversal itself: every CA
-step after the first replays the previous step's traversal, and each
-replayed pair whose replayed subtree holds a leaf pair becomes a sample --
-its 32 features (c2a_amd/csrc/scorer.h) and the label
-log(closest replayed triangle distance in that subtree + 1e-4).  Splits are
-by query, by hashing the query index with a seed, so no query contributes to
-both sides.  Training runs the project's own matrix-core forward/backward
-kernels (c2a_scorer_train).  The scorer only orders siblings; it never
-prunes a pair, so every result stays a conservative-advancement result.
+The data come from the GPU traversal itself: every CA step after the first
+replays the previous step's traversal, and each replayed pair whose
+replayed subtree holds a leaf pair becomes a sample -- its 32 features
+(c2a_amd/csrc/scorer.h) and the label log(closest replayed triangle distance
+in that subtree + 1e-4).  Training has two stages: a first scorer learns
+from the plain (reference-order) traversal's samples; then the traversal
+runs in that scorer's order, and the final scorer learns from both sets, so
+it also sees the pairs its own order visits.  Splits are by query, by
+hashing the query index with a seed, so no query contributes to both sides.
+Training runs the project's own matrix-core forward/backward kernels
+(c2a_scorer_train).  The scorer only orders siblings; it never prunes a
+pair, so every result stays a conservative-advancement result.
 
     python -m c2a_amd.learn --out c2a_amd/scorer_knot_uneven.c2as
 """
@@ -26,10 +29,12 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_SCORER = os.path.join(_HERE, "scorer_knot_uneven.c2as")
 
 
-def collect_samples(meshes, queries, tol_d, tol_t, rate, cap):
-    """One plain-order batch run with sample logging: (X, y, keys, selected)."""
+def collect_samples(meshes, queries, tol_d, tol_t, rate, cap, scorer=None):
+    """One batch run with sample logging, in the plain order or in `scorer`'s
+    learned order: (X, y, keys, selected)."""
+    kw = dict(order=G.ORDER_LEARNED, scorer=scorer) if scorer is not None else {}
     p = G.default_params(tol_d=tol_d, tol_t=tol_t, with_contacts=0, sample_rate=rate,
-                         sample_cap=cap)
+                         sample_cap=cap, **kw)
     b = G.Batch(meshes, queries, p)
     b.run()
     return b.samples(cap)
@@ -122,7 +127,8 @@ def main():
     ap.add_argument("--uneven-nq", type=int, default=16384)
     ap.add_argument("--rate", type=float, default=0.1)
     ap.add_argument("--cap", type=int, default=12_000_000)
-    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=12, help="first-stage epochs")
+    ap.add_argument("--epochs2", type=int, default=16, help="second-stage epochs (0: one stage)")
     ap.add_argument("--split-seed", type=int, default=7)
     ap.add_argument("--seed", type=int, default=1)
     a = ap.parse_args()
@@ -146,6 +152,21 @@ def main():
         runs[name] = (meshes, q, cfg, nq, X[~m], y[~m], k[~m])
     X, y = np.concatenate(Xs), np.concatenate(ys)
     s = train_scorer(X, y, epochs=a.epochs, seed=a.seed)
+    if a.epochs2 > 0:
+        # second stage: samples of the training queries in the first scorer's order
+        for name, (meshes, q, cfg, nq, _, _, _) in runs.items():
+            X2, y2, k2, n2 = collect_samples(meshes, q, cfg["tol_d"], cfg["tol_t"], a.rate, a.cap,
+                                             scorer=s)
+            if n2 > a.cap:
+                raise SystemExit("%s: %d samples selected, above the cap %d (lower --rate)"
+                                 % (name, n2, a.cap))
+            m2 = split_mask(k2, 0.8, a.split_seed)
+            print("%s, learned order: %d samples, %d for training" % (name, n2, m2.sum()),
+                  flush=True)
+            Xs.append(X2[m2])
+            ys.append(y2[m2])
+        X, y = np.concatenate(Xs), np.concatenate(ys)
+        s = train_scorer(X, y, epochs=a.epochs2, seed=a.seed)
     s.save(a.out)
     print("saved", a.out)
     for name, (meshes, q, cfg, nq, Xv, yv, kv) in runs.items():

@@ -78,6 +78,10 @@ typedef struct {
   double sample_rate;       /* fraction of replayed pairs logged as training
                                samples (selected by hashing their key) */
   int64_t sample_cap;       /* rows of the sample buffer; 0 = no logging */
+  int32_t miss_depth;       /* speculative depth of the first round below a
+                               replayed pair the step must expand further
+                               than the last one (default 2; 0 = spec_depth) */
+  int32_t pad;
 } c2a_params;
 
 enum { C2A_ORDER_PLAIN = 0, C2A_ORDER_LEARNED = 1 };
