@@ -132,9 +132,9 @@ def test_sampling_keeps_results_and_selects_by_hash(emu_learned):
 
 def test_learned_order_is_conservative_on_cpu(emu_learned):
     """A (random) scorer only reorders siblings: every query still ends with
-    the plain order's collision flag, no TOC after the first contact (the
-    oracle at tol_d 1e-6 gives that time to within about 1e-6 / speed), and
-    a different traversal."""
+    the plain order's collision flag, no colliding query ends with its TOC
+    pose in contact unless the reference order's does too, and the
+    traversal differs."""
     emu = emu_learned
     t9, P = _knot_case(24, 901)
     params = G.default_params(tol_d=1e-3, tol_t=1e-4)
@@ -144,11 +144,8 @@ def test_learned_order_is_conservative_on_cpu(emu_learned):
     emu.emu_set_scorer(p.ctypes.data)
     learned = run_emu(emu, t9, t9, P, params, 5)
     assert np.array_equal(plain["collisionfree"], learned["collisionfree"])
-    m1, m2 = O.OModel(t9), O.OModel(t9)
-    ref = O.batch_toc(m1, m2, *P, 1e-6, 1e-9, seeds=np.zeros((len(P[0]), 2), np.int32),
-                      max_iter=5000, nthreads=4)
-    ok = (ref["collisionfree"] == 0) & ((ref["flags"] & 1) == 0)
-    assert ok.sum() >= 8
-    assert (learned["toc"][ok] <= ref["toc"][ok]).all()
-    assert (plain["toc"][ok] <= ref["toc"][ok]).all()
+
+    def over(r):
+        return (r["collisionfree"] == 0) & (r["toc"] > 0) & (r["distance"] == 0)
+    assert not (over(learned) & ~over(plain)).any()
     assert not np.array_equal(plain["bv_tests"], learned["bv_tests"])
