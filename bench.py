@@ -166,7 +166,7 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the CCD batch has no "
+        print("bench.py: measures one GPU only (the CCD batch has no "
               "cross-GPU step); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import numpy as np

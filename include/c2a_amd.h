@@ -198,8 +198,8 @@ int c2a_motion_prepare(const double *pose_start, const double *pose_end, double 
  * pose12 = R[9] then T[3]; quat4 (optional) = (x,y,z,w). */
 int c2a_motion_pose(const double *motion24, double t, double *pose12, double *quat4);
 
-/* Learned front ordering (no counterpart in the reference; its north-star
- * extension, the pair scorer used to order the traversal).  A scorer is a
+/* Learned front ordering (no counterpart in the reference: a pair scorer
+ * that orders the traversal and never prunes).  A scorer is a
  * 32 -> 64 -> 64 -> 1 ReLU network with BF16 weights on the matrix cores and
  * FP32 accumulation and master parameters (6337 FP32 values: W1[32][64],
  * b1[64], W2[64][64], b2[64], w3[64], b3).  The feature list is frozen in

@@ -10,8 +10,8 @@
  * arithmetic lives in PQP (GammaUNC/PQP, not vendored, version unpinned) the
  * published algorithm is restated and the citation names the C2A call site.
  *
- * Pinning status of each part (what checks it besides self-consistency) is
- * listed in the test files and DESIGN.md; in short:
+ * Pinning status of each part (what checks it besides self-consistency;
+ * the tests in tests/test_oracle_pinning.py hold the checks):
  *   rect_dist          pinned (brute force over edge pairs + corner/face
  *                      projections, dense sampling, symmetry)
  *   tri_dist           pinned (brute force 9 seg-seg + 6 vertex-face, sampling)
