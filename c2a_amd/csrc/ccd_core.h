@@ -220,6 +220,7 @@ struct Wave {
   int32_t anc[CAP + 2];      // scan: log index of the last event at each depth
   int32_t lastlane[CAP + 2]; // scan: scratch for the per-depth last event
   int32_t nleafq;            // replay: leaf records queued for the triangle pass
+  int32_t next;              // replay: pairs queued for the speculative extension
   int64_t ticket;            // work-queue ticket shared by the wavefronts
   int32_t rarena;            // the pending round writes its subtree to the arena
   int32_t rdepth;            // levels of the slot heap the current round filled
@@ -751,6 +752,54 @@ C2A_HD void write_samples(const LeaderT& L, const KArgs& a, const ArenaView& av,
   }
 }
 
+// One child record of expansion e (side 0: first child node, 1: its
+// sibling) at this step's pose: transform from the parent record, pair
+// values, features, and a queue entry for the triangle pass if it is a leaf
+// pair.  `fresh`: the record lies beyond the previous step's records (its
+// child link starts empty).
+template <int K, int CAP>
+C2A_HD void replay_record(Wave<K, CAP>& g, const Leader& L, const KArgs& a, const ArenaView& av,
+                          node_ptr N1, node_ptr N2, const LogEnt le, int e, int side, bool fresh) {
+  double C2A_GLOBAL* xf = gp(av.xf);
+  int32_t C2A_GLOBAL* cpa = gp(av.cpa);
+  HotRec C2A_GLOBAL* hot = gp(av.hot);
+  int32_t C2A_GLOBAL* leafq = gp(av.pos);
+  const int rec = 2 * e + 1 + side;
+  double R[9], T[3];
+  const double C2A_GLOBAL* pxf = xf + 12 * (int64_t)le.rec;
+  for (int i = 0; i < 9; i++) R[i] = pxf[i];
+  for (int i = 0; i < 3; i++) T[i] = pxf[9 + i];
+  int b1 = le.b1, b2 = le.b2;
+  descend(N1, N2, b1, b2, side, R, T);   // an expanded pair is never a leaf pair
+  double d, mt, mb[2] = {0.0, 0.0};
+  const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], false, d, mt, mb);
+  if (a.feats) {
+    uint16_t f[NF];
+    pair_feature_row(f, N1, N2, b1, b2, R, T, d, mt, mb, g.mo[0], g.mo[1], L.Up, fl);
+    uint16_t C2A_GLOBAL* of = gp(av.feat) + (int64_t)rec * NF;
+    for (int k = 0; k < NF; k++) of[k] = f[k];
+  }
+  double C2A_GLOBAL* oxf = xf + 12 * (int64_t)rec;
+  for (int i = 0; i < 9; i++) oxf[i] = R[i];
+  for (int i = 0; i < 3; i++) oxf[9 + i] = T[i];
+  HotRec h;
+  h.d = d;
+  h.mt = mt;
+  h.b1 = b1;
+  h.b2 = b2;
+  h.flags = fl;
+  if (fresh) cpa[rec] = -1;
+  h.cp = fresh ? -1 : cpa[rec];
+  h.dTri = 0.0;
+  h.mleaf = 0.0;
+  h.t1 = -1;
+  h.t2 = -1;
+  h.score = 0.0f;
+  h.pad = 0;
+  hot[rec] = h;
+  if (fl == 3) leafq[lds_add(&g.nleafq, 1)] = rec;
+}
+
 // Recomputes, at this step's poses, every pair the previous step's walk
 // computed as a child of an expansion (C2A.cpp:1195-1276 per pair), level by
 // level so that each record's parent transform is ready before it is needed.
@@ -805,43 +854,79 @@ C2A_HD void replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl)
     const int cnt = (v + 1 < CAP + 2 ? g.ls[v + 1] : n) - s0;
     for (int j = lane; j < 2 * cnt; j += nl) {
       const int e = order[s0 + (j >> 1)];
-      const int side = j & 1;
-      const LogEnt le = lg[e];
-      const int rec = 2 * e + 1 + side;
-      double R[9], T[3];
-      const double C2A_GLOBAL* pxf = xf + 12 * (int64_t)le.rec;
-      for (int i = 0; i < 9; i++) R[i] = pxf[i];
-      for (int i = 0; i < 3; i++) T[i] = pxf[9 + i];
-      int b1 = le.b1, b2 = le.b2;
-      descend(N1, N2, b1, b2, side, R, T);   // an expanded pair is never a leaf pair
-      double d, mt, mb[2] = {0.0, 0.0};
-      const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], false, d, mt, mb);
-      if (a.feats) {
-        uint16_t f[NF];
-        pair_feature_row(f, N1, N2, b1, b2, R, T, d, mt, mb, g.mo[0], g.mo[1], L.Up, fl);
-        uint16_t C2A_GLOBAL* of = gp(av.feat) + (int64_t)rec * NF;
-        for (int k = 0; k < NF; k++) of[k] = f[k];
-      }
-      double C2A_GLOBAL* oxf = xf + 12 * (int64_t)rec;
-      for (int i = 0; i < 9; i++) oxf[i] = R[i];
-      for (int i = 0; i < 3; i++) oxf[9 + i] = T[i];
-      HotRec h;
-      h.d = d;
-      h.mt = mt;
-      h.b1 = b1;
-      h.b2 = b2;
-      h.flags = fl;
-      h.cp = cpa[rec];
-      h.dTri = 0.0;
-      h.mleaf = 0.0;
-      h.t1 = -1;
-      h.t2 = -1;
-      h.score = 0.0f;
-      h.pad = 0;
-      hot[rec] = h;
-      if (fl == 3) leafq[lds_add(&g.nleafq, 1)] = rec;
+      replay_record(g, L, a, av, N1, N2, lg[e], e, j & 1, false);
     }
     mem_sync();
+  }
+  // Speculative extension: a replayed pair the previous step did not expand
+  // but whose values now pass the descend test at this step's initial bound
+  // may be expanded by this step (the bound only tightens, so pairs that fail
+  // it now will be pruned).  Their children -- and, level by level, the
+  // children of those that pass too -- are computed here in parallel as
+  // pseudo-expansions appended to the log, so that the scan finds them
+  // replayed instead of walking below them one round at a time.  Values never
+  // depend on the bound, so this changes no result.
+  int ntot = n;
+  if (a.ext_levels > 0) {
+    LogEnt C2A_GLOBAL* lgw = const_cast<LogEnt C2A_GLOBAL*>(lg);
+    int lo = 1, hi = nrec;
+    for (int pass = 0; pass < a.ext_levels; pass++) {
+      if (lane == 0) g.next = 0;
+      lane_sync();
+      const int room = a.logcap - ntot;
+      for (int r = lo + lane; r < hi; r += nl) {
+        const HotRec h = hot[r];
+        if ((h.flags & 3) == 3 || cpa[r] >= 0) continue;
+        const bool desc =
+            h.mt < L.Up && ((h.d < (L.D - L.abs_err)) || (h.d * (1 + L.rel_err) < L.D));
+        if (!desc) continue;
+        const int k = lds_add(&g.next, 1);
+        if (k >= room) continue;   // no room: the scan walks below it
+        const int e = ntot + k;
+        LogEnt v;
+        v.rec = r;
+        v.b1 = h.b1;
+        v.b2 = h.b2;
+        v.level = lgw[(r - 1) >> 1].level + 1;   // depth of record r
+        lgw[e] = v;
+        cpa[r] = e;
+        hot[r].cp = e;
+      }
+      mem_sync();
+      const int cnt = uni(g.next) < room ? uni(g.next) : room;
+      if (cnt <= 0) break;
+      for (int j = lane; j < 2 * cnt; j += nl) {
+        const int e = ntot + (j >> 1);
+        replay_record(g, L, a, av, N1, N2, lgw[e], e, j & 1, true);
+      }
+      mem_sync();
+      lo = 2 * ntot + 1;
+      hi = 2 * (ntot + cnt) + 1;
+      ntot += cnt;
+    }
+    if (ntot > n) {
+      // level order over the logged and the pseudo-expansions for the layout
+      for (int i = lane; i < CAP + 2; i += nl) g.lh[i] = 0;
+      lane_sync();
+      for (int e = lane; e < ntot; e += nl) lds_add(&g.lh[lgw[e].level], 1);
+      lane_sync();
+      if (lane == 0) {
+        int acc = 0, tp = 0;
+        for (int v = 0; v < CAP + 2; v++) {
+          g.ls[v] = acc;
+          const int c = g.lh[v];
+          if (c) tp = v + 1;
+          g.lh[v] = acc;
+          acc += c;
+        }
+        g.ls[CAP + 2] = tp;
+      }
+      lane_sync();
+      for (int e = lane; e < ntot; e += nl) order[lds_add(&g.lh[lgw[e].level], 1)] = e;
+      mem_sync();
+      L.nlog_prev = ntot;   // the layout and the scan cover the extension too
+      L.napp = ntot;        // arena rounds append after the extension
+    }
   }
   // triangle distances of the leaf pairs, in dense chunks (C2A.cpp:1141-1166)
   const int nleaf = g.nleafq;

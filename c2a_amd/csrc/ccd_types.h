@@ -169,6 +169,9 @@ struct KArgs {
   unsigned char* arena;    // one arena_bytes(logcap, feats) slice per workgroup
   int32_t feats;           // the arena holds features (learned order or samples)
   int32_t miss_depth;      // depth of the first round below a miss (0 = K)
+  int32_t ext_levels;      // replay: levels of speculative extension below
+                           // pairs that newly pass the descend test
+  int32_t pad3;
   ScorerDev scorer;        // learned child order when scorer.on
   // training samples of replayed pairs: a pair is logged when the top 32
   // bits of mix64(key) are below sample_thr (2^32 = every pair)

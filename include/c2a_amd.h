@@ -81,7 +81,11 @@ typedef struct {
   int32_t miss_depth;       /* speculative depth of the first round below a
                                replayed pair the step must expand further
                                than the last one (default 2; 0 = spec_depth) */
-  int32_t pad;
+  int32_t ext_levels;       /* levels computed ahead, during the replay, below
+                               replayed pairs that newly pass the descend test
+                               at the step's initial bound (default 0 = off:
+                               the extra replayed records cost more than the
+                               rounds they save on the measured workloads) */
 } c2a_params;
 
 enum { C2A_ORDER_PLAIN = 0, C2A_ORDER_LEARNED = 1 };

@@ -63,3 +63,18 @@ def test_solve_tolerances_bit_exact(knot_pair):
     got = G.ccd_batch(meshes, G.make_queries(*P, seeds=seeds), G.default_params())
     ref = O.batch_toc(m1, m2, *P, 1e-4, 1e-4, seeds=seeds, with_contacts=True, nthreads=8)
     assert_same(ref, got, "solve")
+
+
+@pytest.mark.parametrize("knob", ["ext_levels=2", "miss_depth=5", "replay_log=0"])
+def test_traversal_knobs_bit_exact(knot_pair, knob):
+    """Speculation and replay settings change the work, never the result."""
+    t9, meshes, m1, m2 = knot_pair
+    nq = 128
+    name, val = knob.split("=")
+    print("knot batch nq=%d %s" % (nq, knob), flush=True)
+    P = W.knot_queries(nq, 4242, collide_frac=0.3)
+    got = G.ccd_batch(meshes, G.make_queries(*P),
+                      G.default_params(tol_d=1e-3, tol_t=1e-4, **{name: int(val)}))
+    ref = O.batch_toc(m1, m2, *P, 1e-3, 1e-4, seeds=np.zeros((nq, 2), np.int32),
+                      with_contacts=True, nthreads=8)
+    assert_same(ref, got, knob)
