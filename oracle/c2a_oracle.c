@@ -1260,7 +1260,240 @@ void c2o_toc_step(const c2o_model *m1, const c2o_model *m2,
   if (upd2) upd2->last_tri = cx.last2;
 }
 
-/* C2A_QueryTimeOfContact (C2A.cpp:1987-2146), rotation path. */
+/* ------------------------------------------------------------------------ */
+/* Translation-only CA (C2A_Solve when both rotations are below 1e-8, the    */
+/* global b_TanslationCCD): C2A.cpp:1362-1521 (traversal), 1818-1852 (seed   */
+/* bound), 1900-1917 (final pose), 2032-2049 (toc = mint, no CA loop);       */
+/* InterpMotion.cpp:289-298 (ConservD), 570-655 (CAonRSS), 657-743           */
+/* (CAonNonAdjacentTriangles).  Only object 1's velocity enters: object 2 is */
+/* treated as static.  m_toc_delta is C2A_Solve's d_delta (the query's       */
+/* tolerance_d, C2A.cpp:2384-2388).                                          */
+/* ------------------------------------------------------------------------ */
+static int g_solve_translation = 1;
+void c2o_set_solve_translation(int on) { g_solve_translation = on; }
+
+/* ConservD (InterpMotion.cpp:289-298): GMP_CCD_SECURITY_DISTANCE_RATIO 0.1. */
+static double conserv_d(double d, double delta) { return d - 0.1 * delta; }
+
+/* CAonRSS (InterpMotion.cpp:570-655) for BV pair (b1, b2) at transform
+ * (R, T): local conservative advancement of the pair under object 1's
+ * translation, capped at *mint.  The reference reads an uninitialised
+ * direction when the two rectangles overlap at the start (C2ARectDist leaves
+ * S unset); here such a pair is touching now: time 0, distance 0.  The
+ * bValid_C_clo == false branch (:607-617) is dead: C2ARectDist always
+ * reports valid closest points. */
+static int ca_on_rss(const c2o_motion *mo1, const m3 r1, const m3 R, const double T[3],
+                     const c2o_bv *b1, const c2o_bv *b2, double delta, double *mint,
+                     double *distance) {
+  double S[3] = {0.0, 0.0, 0.0}, tmp[3], Sw[3], Vel[3], T2[3];
+  double d = bv_distance(R, T, b1, b2, S);
+  mtxv(tmp, r1, mo1->cv);
+  mtxv(Vel, b1->R_loc, tmp);
+  if (d == 0.0) {
+    *mint = 0.0;
+    *distance = 0.0;
+    return 1;
+  }
+  mxv(tmp, b1->R_loc, S);
+  mxv(Sw, r1, tmp);
+  double tocf = conserv_d(d, delta) / c2o_motion_bound_leaf(mo1, b1->angR, Sw);
+  double total = tocf;
+  int it = 1;
+  while ((d >= delta) && (total <= mint[0]) && (it < 50)) {
+    for (int i = 0; i < 3; i++) T2[i] = T[i] + Vel[i] * (-total);
+    d = bv_distance(R, T2, b1, b2, S);
+    if (d == 0) break;
+    mxv(tmp, b1->R_loc, S);
+    mxv(Sw, r1, tmp);
+    tocf = conserv_d(d, delta) / c2o_motion_bound_leaf(mo1, b1->angR, Sw);
+    it++;
+    if (tocf < delta) break;
+    total += tocf;
+  }
+  if (total < 1.0) {
+    *mint = total;
+    double dd = (total - delta) * len3(mo1->cv);
+    *distance = dd < 0 ? 0 : dd;
+    return 1;
+  }
+  return 0;
+}
+
+/* CAonNonAdjacentTriangles (InterpMotion.cpp:657-743): triangle A (model-1
+ * frame) advancing with object 1's velocity against triangle B (already in
+ * model-1 frame). */
+static int ca_on_tris(const c2o_motion *mo1, const m3 r1, const double A[3][3],
+                      const double B[3][3], double delta, double *mint, double *distance) {
+  double Vel[3], p[3], q[3], n[3], At[3][3];
+  mtxv(Vel, r1, mo1->cv);
+  double d = c2o_tri_dist(p, q, A, B, NULL, NULL, NULL, NULL);
+  sub3(n, q, p);
+  normalize3(n);
+  double u = dot3(Vel, n);
+  if (u <= 0) u = 1e-30;
+  if (d == 0) {
+    *mint = 0.0;
+    *distance = 0.0;
+    return 1;
+  }
+  double dt = conserv_d(d, delta) / u;
+  int it = 1;
+  if (dt >= mint[0]) return 0;
+  double total = dt;
+  while ((d > delta) && (total <= mint[0]) && (it < 50)) {
+    for (int v = 0; v < 3; v++)
+      for (int i = 0; i < 3; i++) At[v][i] = A[v][i] + Vel[i] * total;
+    d = c2o_tri_dist(p, q, At, B, NULL, NULL, NULL, NULL);
+    if (d == 0.0) break;
+    sub3(n, q, p);
+    normalize3(n);
+    u = dot3(Vel, n);
+    if (u <= 0) u = 1e-30;
+    double tofc = conserv_d(d, delta) / u;
+    it++;
+    if (tofc < delta) break;
+    total += tofc;
+  }
+  if (total <= mint[0] && total >= 0.0) {
+    *mint = total;
+    *distance = total * len3(mo1->cv) + delta;
+    return 1;
+  }
+  return 0;
+}
+
+typedef struct {
+  const c2o_model *m1, *m2;
+  const c2o_motion *mo1;
+  const double (*r1)[3];
+  m3 Rrel;
+  double Trel[3];
+  double delta;
+  c2o_result *res;
+  int la, lb;                /* res->last_triA/B */
+} trans_ctx;
+
+/* Triangle t of model 2 in model 1's frame (MxVpV, C2A.cpp:1403-1405). */
+static void tri_in_frame1(double out[3][3], const m3 R, const double T[3], const c2o_tri *t) {
+  mxvpv(out[0], R, t->p[0], T);
+  mxvpv(out[1], R, t->p[1], T);
+  mxvpv(out[2], R, t->p[2], T);
+}
+
+static void tri_copy(double out[3][3], const c2o_tri *t) { memcpy(out, t->p, sizeof(t->p)); }
+
+/* TOCStepRecurse_Dis_Translation (C2A.cpp:1362-1521). */
+static void trans_recurse(trans_ctx *cx, const m3 R, const double T[3], int b1, int b2) {
+  const c2o_bv *B1 = &cx->m1->bvs[b1], *B2 = &cx->m2->bvs[b2];
+  c2o_result *res = cx->res;
+  int l1 = B1->first_child < 0, l2 = B2->first_child < 0;
+  if (l1 && l2) {
+    int t1 = -B1->first_child - 1, t2 = -B2->first_child - 1;
+    double A[3][3], Bt[3][3];
+    tri_copy(A, &cx->m1->tris[t1]);
+    tri_in_frame1(Bt, cx->Rrel, cx->Trel, &cx->m2->tris[t2]);
+    if (ca_on_tris(cx->mo1, cx->r1, A, Bt, cx->delta, &res->mint, &res->distance)) {
+      cx->la = t1;
+      cx->lb = t2;
+    }
+    res->num_tri_tests++;
+    return;
+  }
+  int a1, a2, c1, c2;
+  m3 R1, R2;
+  double T1[3], T2[3], Tt[3];
+  if (l2 || (!l1 && (B1->size > B2->size))) {
+    a1 = B1->first_child; c1 = a1 + 1; a2 = c2 = b2;
+    const c2o_bv *A = &cx->m1->bvs[a1], *C = &cx->m1->bvs[c1];
+    mtxm(R1, A->R, R);
+    sub3(Tt, T, A->Tr);
+    mtxv(T1, A->R, Tt);
+    mtxm(R2, C->R, R);
+    sub3(Tt, T, C->Tr);
+    mtxv(T2, C->R, Tt);
+  } else {
+    a1 = c1 = b1; a2 = B2->first_child; c2 = a2 + 1;
+    const c2o_bv *A = &cx->m2->bvs[a2], *C = &cx->m2->bvs[c2];
+    mxm(R1, R, A->R);
+    mxvpv(T1, R, A->Tr, T);
+    mxm(R2, R, C->R);
+    mxvpv(T2, R, C->Tr, T);
+  }
+  double d1 = 1e+30, d2 = 1e+30, minta = res->mint, mintc = res->mint;
+  ca_on_rss(cx->mo1, cx->r1, R1, T1, &cx->m1->bvs[a1], &cx->m2->bvs[a2], cx->delta, &minta, &d1);
+  ca_on_rss(cx->mo1, cx->r1, R2, T2, &cx->m1->bvs[c1], &cx->m2->bvs[c2], cx->delta, &mintc, &d2);
+  res->num_bv_tests += 2;
+  if (d2 < d1) {
+    if (mintc < res->mint &&
+        ((d2 < (res->distance - res->abs_err)) || (d2 * (1 + res->rel_err) < res->distance)))
+      trans_recurse(cx, R2, T2, c1, c2);
+    if (minta < res->mint &&
+        ((d1 < (res->distance - res->abs_err)) || (d1 * (1 + res->rel_err) < res->distance)))
+      trans_recurse(cx, R1, T1, a1, a2);
+  } else {
+    if (minta < res->mint &&
+        ((d1 < (res->distance - res->abs_err)) || (d1 * (1 + res->rel_err) < res->distance)))
+      trans_recurse(cx, R1, T1, a1, a2);
+    if (mintc < res->mint &&
+        ((d2 < (res->distance - res->abs_err)) || (d2 * (1 + res->rel_err) < res->distance)))
+      trans_recurse(cx, R2, T2, c1, c2);
+  }
+}
+
+/* The translation-only query: one step from the start poses, then
+ * toc = mint (C2A.cpp:2032-2049).  The reference copies uninitialised
+ * closest points into res->p1/p2 (C2A.cpp:1411-1412); here they are the
+ * closest points of the final triangle distance.  The models' last_tri are
+ * not updated (the branch writes res->last_triA/B only). */
+static double query_toc_translation(c2o_model *m1, c2o_model *m2, c2o_motion *mo1,
+                                    c2o_motion *mo2, c2o_result *res, double delta) {
+  trans_ctx cx;
+  m3 R, R1, R2;
+  double T[3], T1[3], T2[3], Ttmp[3];
+  memcpy(R1, mo1->R0, sizeof(m3));
+  memcpy(T1, mo1->T0, sizeof(T1));
+  memcpy(R2, mo2->R0, sizeof(m3));
+  memcpy(T2, mo2->T0, sizeof(T2));
+  cx.m1 = m1; cx.m2 = m2; cx.mo1 = mo1; cx.r1 = R1; cx.delta = delta; cx.res = res;
+  root_transform(m1, m2, R1, T1, R2, T2, cx.Rrel, cx.Trel, R, T);
+  cx.la = res->last_triA;
+  cx.lb = res->last_triB;
+  /* seed bound from the caller's triangle pair (C2A.cpp:1818-1852) */
+  double A[3][3], B[3][3], mint = 1.0, dTri = 0.0;
+  tri_copy(A, &m1->tris[cx.la]);
+  tri_in_frame1(B, cx.Rrel, cx.Trel, &m2->tris[cx.lb]);
+  if (ca_on_tris(mo1, R1, A, B, delta, &mint, &dTri)) {
+    res->mint = mint;
+    res->distance = dTri;
+  } else {
+    res->mint = 1.0;
+    res->distance = 1e+30;
+  }
+  res->abs_err = 0;
+  res->rel_err = 0;
+  trans_recurse(&cx, R, T, 0, 0);
+  /* distance at the advanced pose (C2A.cpp:1906-1916) */
+  c2o_motion_integrate(mo1, res->mint, R1, T1, NULL);
+  c2o_motion_integrate(mo2, res->mint, R2, T2, NULL);
+  m3 Rr;
+  double Tr[3];
+  mtxm(Rr, R1, R2);
+  sub3(Ttmp, T2, T1);
+  mtxv(Tr, R1, Ttmp);
+  res->distance = tri_distance(Rr, Tr, &m1->tris[cx.la], &m2->tris[cx.lb], res->p1, res->p2,
+                               NULL, NULL, NULL, NULL);
+  res->toc = res->mint;
+  res->collisionfree = res->toc >= 1.0 ? 1 : 0;
+  res->numCA = 0;
+  if (!res->collisionfree) c2o_motion_integrate(mo1, res->toc, res->R_toc, res->T_toc, NULL);
+  res->out_last_triA = m1->last_tri;
+  res->out_last_triB = m2->last_tri;
+  return res->toc;
+}
+
+/* C2A_QueryTimeOfContact (C2A.cpp:1987-2146); with both rotations below
+ * 1e-8 and the translation switch on (C2A_Solve's b_TanslationCCD), the
+ * translation-only branch. */
 double c2o_query_toc(c2o_model *m1, c2o_model *m2, c2o_motion *mo1,
                      c2o_motion *mo2, c2o_result *res, double tol_d,
                      double tol_t, int max_iter) {
@@ -1279,6 +1512,8 @@ double c2o_query_toc(c2o_model *m1, c2o_model *m2, c2o_motion *mo1,
   double lamda = 0.0, lastLamda = 0.0, dist, mint;
   int nItrs = 0;
   if (mo1->angVel < 1e-8 && mo2->angVel < 1e-8) res->flags |= C2O_FLAG_TRANSLATION_ONLY;
+  if (g_solve_translation && (res->flags & C2O_FLAG_TRANSLATION_ONLY))
+    return query_toc_translation(m1, m2, mo1, mo2, res, tol_d);
 
   c2o_toc_step(m1, m2, mo1, mo2, R1, T1, R2, T2, m1, m2, res);
   dist = res->distance;

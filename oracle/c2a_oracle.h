@@ -90,8 +90,8 @@ typedef struct {
 
 enum {
   C2O_FLAG_ITER_CAP = 1,        /* hit the 150-iteration cap (C2A.cpp:2079) */
-  C2O_FLAG_TRANSLATION_ONLY = 2 /* both angVel < 1e-8: translation CA path,
-                                   not restated (C2A.cpp:1818-1852) */
+  C2O_FLAG_TRANSLATION_ONLY = 2 /* both angVel < 1e-8: the translation-only
+                                   branch ran (C2A.cpp:1362-1521, 1818-1852) */
 };
 
 /* Contact record produced by the contact enumeration (ContactF, C2A.h:116-153). */
@@ -146,10 +146,13 @@ void c2o_toc_step(const c2o_model *m1, const c2o_model *m2,
 
 /* C2A_QueryTimeOfContact (C2A.cpp:1987-2146).  Seeds the first-step bound
  * with (res->last_triA, res->last_triB).  Updates m*->last_tri like the
- * reference.  Returns toc. */
+ * reference.  Returns toc.  With both rotations below 1e-8 it takes
+ * C2A_Solve's translation-only branch (toc = mint after one traversal)
+ * unless c2o_set_solve_translation(0). */
 double c2o_query_toc(c2o_model *m1, c2o_model *m2, c2o_motion *mo1,
                      c2o_motion *mo2, c2o_result *res, double tol_d,
                      double tol_t, int max_iter);
+void c2o_set_solve_translation(int on);   /* default 1 (C2A_Solve semantics) */
 
 /* Contact enumeration at absolute poses (C2A_QueryContact,
  * C2A.cpp:1937-1966 -> 1727-1775 -> 1525-1725).  Returns the count; fills

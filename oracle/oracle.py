@@ -86,6 +86,7 @@ def lib():
         L.c2o_motion_bound_leaf.argtypes = [C.POINTER(Motion), C.c_double, dp]
         L.c2o_sincos.argtypes = [C.c_double, dp, dp]
         L.c2o_set_libm_trig.argtypes = [C.c_int]
+        L.c2o_set_solve_translation.argtypes = [C.c_int]
         L.c2o_model_build.restype = C.c_int
         L.c2o_model_build.argtypes = [C.POINTER(Model), C.c_int, dp, ip, ip]
         L.c2o_model_free.argtypes = [C.POINTER(Model)]
@@ -163,6 +164,12 @@ def sincos(x):
 
 def set_libm_trig(flag):
     lib().c2o_set_libm_trig(int(flag))
+
+
+def set_solve_translation(flag):
+    """C2A_Solve's translation-only branch for queries whose two rotations are
+    below 1e-8 (default on)."""
+    lib().c2o_set_solve_translation(int(flag))
 
 
 def jacobi_eigen(A):
