@@ -37,7 +37,8 @@ class Params(C.Structure):
                 ("contact_offset", C.c_double), ("spec_depth", C.c_int32),
                 ("waves_per_query", C.c_int32), ("replay_log", C.c_int32), ("order", C.c_int32),
                 ("scorer", C.c_void_p), ("sample_rate", C.c_double), ("sample_cap", C.c_int64),
-                ("miss_depth", C.c_int32), ("ext_levels", C.c_int32)]
+                ("miss_depth", C.c_int32), ("ext_levels", C.c_int32),
+                ("translation", C.c_int32), ("pad", C.c_int32)]
 
 
 class Result(C.Structure):

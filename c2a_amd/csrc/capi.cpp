@@ -230,6 +230,7 @@ int fill_args(c2a_batch* b, const c2a_params* p) {
   a.feats = (b->scorer || b->sample_cap > 0) ? 1 : 0;
   a.miss_depth = p->miss_depth < 0 ? 0 : p->miss_depth;
   a.ext_levels = p->ext_levels < 0 ? 0 : (p->ext_levels > 8 ? 8 : p->ext_levels);
+  a.translation = p->translation != 0 ? 1 : 0;
   if (b->sample_cap > 0) {
     const double r = p->sample_rate < 0 ? 0.0 : (p->sample_rate > 1 ? 1.0 : p->sample_rate);
     a.sample_thr = (uint64_t)(r * 4294967296.0);
@@ -274,6 +275,8 @@ void c2a_params_default(c2a_params* p) {
   p->sample_cap = 0;
   p->miss_depth = 2;
   p->ext_levels = 0;
+  p->translation = 1;
+  p->pad = 0;
 }
 
 int c2a_mesh_create(const double* tri_xyz9, const int32_t* tri_ids, const int32_t* vert_idx3,

@@ -250,7 +250,8 @@ enum : int {
   ST_FETCH = 0, ST_STEP = 1, ST_WALK = 2, ST_CONTACT_WALK = 3, ST_DONE = 4,
   ST_SCAN = 5,   // lane-parallel walk over the replayed subtree
   ST_SUB = 6,    // sequential walk below a pair the scan found unexpanded
-  ST_LAYOUT = 7  // replayed records scored: lay them out and scan
+  ST_LAYOUT = 7, // replayed records scored: lay them out and scan
+  ST_TRANS = 8   // translation-only query (C2A_Solve, both rotations < 1e-8)
 };
 
 // The CA loop and step state of one query; every lane holds the same values.
@@ -415,6 +416,101 @@ C2A_HD void pair_feature_row(uint16_t* out, node_ptr N1, node_ptr N2, int b1, in
   pair_features(f, l1, l2, A1->r, A2->r, R, T, d, mt, mb[0], mb[1], A1->depth, A2->depth,
                 A1->angR, A2->angR, mo1, mo2, up, fl & 3);
   for (int k = 0; k < NF; k++) out[k] = f32_to_bf16(f[k]);
+}
+
+// Translation-only conservative advancement (C2A_Solve when both rotations
+// are below 1e-8): InterpMotion.cpp:289-298 (ConservD, security ratio 0.1),
+// 570-655 (CAonRSS), 657-743 (CAonNonAdjacentTriangles).  Only object 1's
+// velocity enters; `delta` is C2A_Solve's m_toc_delta (its tolerance_d).
+C2A_HD double conserv_d(double d, double delta) { return d - 0.1 * delta; }
+
+// CAonRSS for node pair (A1, A2) at transform (R, T).  Rectangles that
+// overlap at the start leave the reference's direction unset; such a pair is
+// touching now (time 0, distance 0).
+C2A_HD bool ca_on_rss(const Motion& mo1, const double* r1, const double* R, const double* T,
+                      node_ptr A1, node_ptr A2, double delta, double& mint, double& distance) {
+  double S[3] = {0.0, 0.0, 0.0}, tmp[3], Sw[3], Vel[3], T2[3], RL[9];
+  for (int i = 0; i < 9; i++) RL[i] = A1->Rloc[i];
+  const double la[2] = {A1->l[0], A1->l[1]}, lb[2] = {A2->l[0], A2->l[1]};
+  const double rr = A1->r + A2->r;
+  const double angR = A1->angR;
+  double d = rect_dist(R, T, la, lb, S);
+  d -= rr;
+  d = (d < 0.0) ? 0.0 : d;
+  mtxv(tmp, r1, mo1.cv);
+  mtxv(Vel, RL, tmp);
+  if (d == 0.0) {
+    mint = 0.0;
+    distance = 0.0;
+    return true;
+  }
+  mxv(tmp, RL, S);
+  mxv(Sw, r1, tmp);
+  double tocf = conserv_d(d, delta) / motion_bound_leaf(mo1, angR, Sw);
+  double total = tocf;
+  int it = 1;
+  while ((d >= delta) && (total <= mint) && (it < 50)) {
+    for (int i = 0; i < 3; i++) T2[i] = T[i] + Vel[i] * (-total);
+    d = rect_dist(R, T2, la, lb, S);
+    d -= rr;
+    d = (d < 0.0) ? 0.0 : d;
+    if (d == 0) break;
+    mxv(tmp, RL, S);
+    mxv(Sw, r1, tmp);
+    tocf = conserv_d(d, delta) / motion_bound_leaf(mo1, angR, Sw);
+    it++;
+    if (tocf < delta) break;
+    total += tocf;
+  }
+  if (total < 1.0) {
+    mint = total;
+    const double dd = (total - delta) * len3(mo1.cv);
+    distance = dd < 0 ? 0 : dd;
+    return true;
+  }
+  return false;
+}
+
+// CAonNonAdjacentTriangles: triangle A advancing with object 1's velocity
+// against triangle B, both in model 1's frame (9 doubles each).
+C2A_HD bool ca_on_tris(const Motion& mo1, const double* r1, const double* A, const double* B,
+                       double delta, double& mint, double& distance) {
+  double Vel[3], p[3], q[3], n[3], At[9];
+  mtxv(Vel, r1, mo1.cv);
+  double d = tri_dist(p, q, A, B, nullptr);
+  sub3(n, q, p);
+  normalize3(n);
+  double u = dot3(Vel, n);
+  if (u <= 0) u = 1e-30;
+  if (d == 0) {
+    mint = 0.0;
+    distance = 0.0;
+    return true;
+  }
+  const double dt = conserv_d(d, delta) / u;
+  int it = 1;
+  if (dt >= mint) return false;
+  double total = dt;
+  while ((d > delta) && (total <= mint) && (it < 50)) {
+    for (int v = 0; v < 3; v++)
+      for (int i = 0; i < 3; i++) At[3 * v + i] = A[3 * v + i] + Vel[i] * total;
+    d = tri_dist(p, q, At, B, nullptr);
+    if (d == 0.0) break;
+    sub3(n, q, p);
+    normalize3(n);
+    u = dot3(Vel, n);
+    if (u <= 0) u = 1e-30;
+    const double tofc = conserv_d(d, delta) / u;
+    it++;
+    if (tofc < delta) break;
+    total += tofc;
+  }
+  if (total <= mint && total >= 0.0) {
+    mint = total;
+    distance = total * len3(mo1.cv) + delta;
+    return true;
+  }
+  return false;
 }
 
 // Heap slot h of the depth-K subtree below X: the pair reached by the path
@@ -1666,8 +1762,137 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   L.Up = 1;
   L.M = 1;
   if (q.mo[0].angVel < 1e-8 && q.mo[1].angVel < 1e-8) L.flags |= C2A_FLAG_TRANSLATION_ONLY;
-  L.state = ST_STEP;
+  L.state = (a.translation && (L.flags & C2A_FLAG_TRANSLATION_ONLY)) ? ST_TRANS : ST_STEP;
   return true;
+}
+
+// Triangle t of a mesh (9 doubles), and the same in model 1's frame.
+C2A_HD void load_tri(double* out, const MeshDev& M, int t) {
+  gdouble_ptr g = tris_of(M) + 9 * (int64_t)t;
+  for (int i = 0; i < 9; i++) out[i] = g[i];
+}
+
+// The translation-only query (C2A.cpp:1818-1852 seed bound, 1362-1521
+// traversal, 1900-1917 final pose, 2032-2049 toc = mint): one traversal from
+// the start poses, depth first with the reference's order and tests, over
+// the LDS stack.  Every lane runs the same sequential code.  Pair values
+// depend on the current mint (CAonRSS stops at it), so they are computed when
+// their parent is expanded, as in the reference.  The reference leaves
+// res->p1/p2 uninitialised on this branch; here they are the closest points
+// of the final triangle distance.  The models' last_tri are not updated.
+template <int K, int CAP>
+C2A_HD void translation_query(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
+  const MeshDev M1 = mesh_of(g, 0);
+  const MeshDev M2 = mesh_of(g, 1);
+  node_ptr N1 = nodes_of(M1);
+  node_ptr N2 = nodes_of(M2);
+  const double delta = a.tol_d;
+  double R1[9], T1[3], R2[9], T2[3];
+  for (int i = 0; i < 9; i++) { R1[i] = g.mo[0].R0[i]; R2[i] = g.mo[1].R0[i]; }
+  for (int i = 0; i < 3; i++) { T1[i] = g.mo[0].T0[i]; T2[i] = g.mo[1].T0[i]; }
+  start_traversal(g, L, M1, M2, R1, T1, R2, T2, -1, lane);
+  double Rrel[9], Trel[3], r1[9];
+  for (int i = 0; i < 9; i++) { Rrel[i] = g.Rrel[i]; r1[i] = R1[i]; }
+  for (int i = 0; i < 3; i++) Trel[i] = g.Trel[i];
+  int la = L.seed1, lb = L.seed2;
+  {
+    double A[9], tb[9], B[9];
+    load_tri(A, M1, la);
+    load_tri(tb, M2, lb);
+    for (int v = 0; v < 3; v++) mxvpv(B + 3 * v, Rrel, tb + 3 * v, Trel);
+    double mint = 1.0, dTri = 0.0;
+    if (ca_on_tris(g.mo[0], r1, A, B, delta, mint, dTri)) {
+      L.M = mint;
+      L.D = dTri;
+    } else {
+      L.M = 1.0;
+      L.D = 1e+30;
+    }
+  }
+  L.abs_err = 0;
+  L.rel_err = 0;
+  while (L.sp > 0) {
+    L.sp--;
+    const Entry& e = g.stack[L.sp];
+    double R[9], T[3];
+    for (int i = 0; i < 9; i++) R[i] = unid(e.R[i]);
+    for (int i = 0; i < 3; i++) T[i] = unid(e.T[i]);
+    const int b1 = uni(e.b1), b2 = uni(e.b2), flags = uni(e.flags);
+    const double dv = unid(e.d), mv = unid(e.mt);
+    lane_sync();
+    if (!(flags & 4)) {   // the parent's deferred test (C2A.cpp:1477-1513)
+      if (!(mv < L.M && ((dv < (L.D - L.abs_err)) || (dv * (1 + L.rel_err) < L.D)))) continue;
+    }
+    node_ptr B1 = N1 + b1;
+    node_ptr B2 = N2 + b2;
+    const bool l1 = node_leaf(B1), l2 = node_leaf(B2);
+    if (l1 && l2) {   // C2A.cpp:1395-1425
+      const int t1 = -B1->first_child - 1, t2 = -B2->first_child - 1;
+      double A[9], tb[9], B[9];
+      load_tri(A, M1, t1);
+      load_tri(tb, M2, t2);
+      for (int v = 0; v < 3; v++) mxvpv(B + 3 * v, Rrel, tb + 3 * v, Trel);
+      if (ca_on_tris(g.mo[0], r1, A, B, delta, L.M, L.D)) {
+        la = t1;
+        lb = t2;
+      }
+      L.tri++;
+      continue;
+    }
+    if (L.sp + 2 > CAP) {
+      L.flags |= C2A_FLAG_STACK_OVERFLOW;
+      break;
+    }
+    // children (C2A.cpp:1435-1466) and their local advancement (1470-1474)
+    double RA[9], TA[3], RC[9], TC[3];
+    for (int i = 0; i < 9; i++) { RA[i] = R[i]; RC[i] = R[i]; }
+    for (int i = 0; i < 3; i++) { TA[i] = T[i]; TC[i] = T[i]; }
+    int a1 = b1, a2 = b2, c1 = b1, c2 = b2;
+    descend(N1, N2, a1, a2, 0, RA, TA);
+    descend(N1, N2, c1, c2, 1, RC, TC);
+    double dA = 1e+30, dC = 1e+30, mA = L.M, mC = L.M;
+    ca_on_rss(g.mo[0], r1, RA, TA, N1 + a1, N2 + a2, delta, mA, dA);
+    ca_on_rss(g.mo[0], r1, RC, TC, N1 + c1, N2 + c2, delta, mC, dC);
+    L.bv += 2;
+    // closer child first (C2A.cpp:1478); both tests wait for their turn
+    const bool c_first = dC < dA;
+    if (lane == 0) {
+      for (int k = 0; k < 2; k++) {
+        const bool second = k == 0;   // pushed first, popped last
+        const bool isC = second ? !c_first : c_first;
+        Entry& o = g.stack[L.sp + k];
+        const double* RR = isC ? RC : RA;
+        const double* TT = isC ? TC : TA;
+        for (int i = 0; i < 9; i++) o.R[i] = RR[i];
+        for (int i = 0; i < 3; i++) o.T[i] = TT[i];
+        o.d = isC ? dC : dA;
+        o.mt = isC ? mC : mA;
+        o.b1 = isC ? c1 : a1;
+        o.b2 = isC ? c2 : a2;
+        o.flags = 0;
+        o.sg = 0;
+      }
+    }
+    lane_sync();
+    L.sp += 2;
+  }
+  L.sp = 0;
+  // distance at the advanced pose (C2A.cpp:1906-1916)
+  motion_integrate(g.mo[0], L.M, R1, T1, nullptr);
+  motion_integrate(g.mo[1], L.M, R2, T2, nullptr);
+  double Tt[3];
+  mtxm(Rrel, R1, R2);
+  sub3(Tt, T2, T1);
+  mtxv(Trel, R1, Tt);
+  {
+    double ta[9], tb[9];
+    load_tri(ta, M1, la);
+    load_tri(tb, M2, lb);
+    L.D = tri_distance(Rrel, Trel, ta, tb, L.p1, L.p2, nullptr);
+  }
+  L.toc = L.M;
+  L.numCA = 0;
+  finish_query(g, L, a, L.toc >= 1.0, lane);
 }
 
 // Advances the query state until a round is needed or no work is left.
@@ -1692,6 +1917,9 @@ C2A_HD void leader_run(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int
         } else {
           L.state = ST_WALK;
         }
+        break;
+      case ST_TRANS:
+        translation_query(g, L, a, lane);
         break;
       case ST_LAYOUT:
         replay_layout(g, L, a, lane, nl);

@@ -171,7 +171,8 @@ struct KArgs {
   int32_t miss_depth;      // depth of the first round below a miss (0 = K)
   int32_t ext_levels;      // replay: levels of speculative extension below
                            // pairs that newly pass the descend test
-  int32_t pad3;
+  int32_t translation;     // C2A_Solve's translation-only branch for queries
+                           // whose two rotations are below 1e-8
   ScorerDev scorer;        // learned child order when scorer.on
   // training samples of replayed pairs: a pair is logged when the top 32
   // bits of mix64(key) are below sample_thr (2^32 = every pair)

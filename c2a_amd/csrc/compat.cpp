@@ -228,6 +228,11 @@ PQP_REAL C2A_QueryTimeOfContact(CInterpMotion* m1, CInterpMotion* m2, C2A_TimeOf
   p.tol_d = tolerance_d;
   p.tol_t = tolerance_t;
   p.with_contacts = 0;
+  // The translation-only branch follows the global b_TanslationCCD, which
+  // only C2A_Solve sets (C2A.cpp:2390-2395); a direct call with that flag
+  // never set takes the rotation path.  (The reference's flag stays set
+  // after a translation-only C2A_Solve; that leftover state is not kept.)
+  p.translation = 0;
   c2a_result r;
   if (c2a_ccd_batch(meshes, 2, &q, 1, &p, &r, nullptr) != C2A_OK) return -1.0;
   copy_result(r, res);

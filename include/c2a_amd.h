@@ -86,6 +86,13 @@ typedef struct {
                                at the step's initial bound (default 0 = off:
                                the extra replayed records cost more than the
                                rounds they save on the measured workloads) */
+  int32_t translation;      /* 1 (default): queries whose two rotations are
+                               below 1e-8 take C2A_Solve's translation-only
+                               branch (C2A.cpp:1362-1521, 2032-2049: one
+                               traversal, toc = mint, object 2's velocity
+                               ignored); 0: the rotation path, as a direct
+                               C2A_QueryTimeOfContact call does */
+  int32_t pad;
 } c2a_params;
 
 enum { C2A_ORDER_PLAIN = 0, C2A_ORDER_LEARNED = 1 };
@@ -111,8 +118,9 @@ typedef struct {
 
 enum {
   C2A_FLAG_ITER_CAP = 1,          /* the 150-iteration cap was hit */
-  C2A_FLAG_TRANSLATION_ONLY = 2,  /* both rotations below 1e-8: the reference
-                                     would take its translation-only branch */
+  C2A_FLAG_TRANSLATION_ONLY = 2,  /* both rotations below 1e-8 (the
+                                     translation-only branch ran when
+                                     params->translation) */
   C2A_FLAG_STACK_OVERFLOW = 4     /* traversal deeper than the stack (never
                                      with capacities chosen from mesh depth) */
 };

@@ -150,6 +150,7 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
   a.arena_rounds = emu_arena_rounds;
   a.miss_depth = p->miss_depth;
   a.ext_levels = p->ext_levels;
+  a.translation = p->translation;
   a.scorer = g_scorer;
   if (!g_keys.empty() && a.logcap > 0) {
     a.sample_thr = (uint64_t)((g_rate < 0 ? 0 : g_rate > 1 ? 1 : g_rate) * 4294967296.0);

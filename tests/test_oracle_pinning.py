@@ -332,7 +332,17 @@ def test_ca_loop_conservative_and_flags(spheres):
     assert seen_hit and seen_free
 
 
-def test_ca_loop_immediate_contact_returns_first_mint(spheres):
+@pytest.fixture()
+def rotation_path():
+    """Pure translations through the rotation path, as a direct
+    C2A_QueryTimeOfContact call takes them (C2A_Solve alone switches to the
+    translation-only branch)."""
+    O.set_solve_translation(0)
+    yield
+    O.set_solve_translation(1)
+
+
+def test_ca_loop_immediate_contact_returns_first_mint(spheres, rotation_path):
     """If the first step already has dist <= tol_d the loop does not run and
     the reported TOC is the first step's mint (C2A.cpp:2072, 2131)."""
     t1, _ = spheres
@@ -344,7 +354,7 @@ def test_ca_loop_immediate_contact_returns_first_mint(spheres):
     assert res.toc == pytest.approx(res.mint) or res.toc == 0.0
 
 
-def test_separating_motion_is_collision_free(spheres):
+def test_separating_motion_is_collision_free(spheres, rotation_path):
     t1, t2 = spheres
     m1, m2 = O.OModel(t1), O.OModel(t2)
     I = np.eye(3)

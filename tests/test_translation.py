@@ -108,20 +108,59 @@ def test_translation_ignores_object2_velocity(balls):
     assert (rr["collisionfree"] == 0).sum() >= 3   # the rotation path sees the hits
 
 
-@pytest.mark.gpu
-def test_gpu_translation_branch_bit_exact(balls):
-    t1, t2 = balls
-    P = [np.concatenate([a, b]) for a, b in zip(translation_queries(48, 7, 1),
-                                                  translation_queries(16, 8, 2))]
-    meshes = [G.Mesh(t1), G.Mesh(t2)]
-    got = G.ccd_batch(meshes, G.make_queries(*P), G.default_params())
-    m1, m2 = O.OModel(t1), O.OModel(t2)
-    ref = O.batch_toc(m1, m2, *P, 1e-4, 1e-4, seeds=np.zeros((64, 2), np.int32),
-                      with_contacts=True, nthreads=8)
+def _both_movers():
+    return [np.concatenate([a, b]) for a, b in zip(translation_queries(48, 7, 1),
+                                                    translation_queries(16, 8, 2))]
+
+
+def _same(ref, got):
     for f, g in [("toc", "toc"), ("distance", "distance"), ("mint", "mint"),
                  ("numCA", "numCA"), ("collisionfree", "collisionfree"),
                  ("num_bv_tests", "bv_tests"), ("num_tri_tests", "tri_tests"),
                  ("num_contact", "num_contact"), ("flags", "flags")]:
         assert np.array_equal(ref[f], got[g]), f
     assert np.array_equal(ref["p1"], got["p1"]) and np.array_equal(ref["p2"], got["p2"])
+    assert np.array_equal(ref["out_last_triA"], got["last_tri"][:, 0])
+    assert np.array_equal(ref["out_last_triB"], got["last_tri"][:, 1])
+
+
+def test_kernel_logic_translation_branch(balls):
+    """The kernel's translation-only query run on the CPU harness."""
+    import ctypes as C
+    import os
+    from tests.test_kernel_logic_cpu import EMU, run_emu
+    if not os.path.exists(EMU):
+        import __graft_entry__
+        __graft_entry__._build_emu()
+    emu = C.CDLL(EMU)
+    t1, t2 = balls
+    P = _both_movers()
+    got = run_emu(emu, t1, t2, P, G.default_params(), 5)
+    m1, m2 = O.OModel(t1), O.OModel(t2)
+    ref = O.batch_toc(m1, m2, *P, 1e-4, 1e-4, seeds=np.zeros((64, 2), np.int32),
+                      with_contacts=True, nthreads=4)
+    _same(ref, got)
     assert (got["collisionfree"] == 0).sum() >= 10
+
+
+@pytest.mark.gpu
+def test_gpu_translation_branch_bit_exact(balls):
+    t1, t2 = balls
+    P = _both_movers()
+    print("translation-only batch: 64 queries", flush=True)
+    meshes = [G.Mesh(t1), G.Mesh(t2)]
+    got = G.ccd_batch(meshes, G.make_queries(*P), G.default_params())
+    m1, m2 = O.OModel(t1), O.OModel(t2)
+    ref = O.batch_toc(m1, m2, *P, 1e-4, 1e-4, seeds=np.zeros((64, 2), np.int32),
+                      with_contacts=True, nthreads=8)
+    _same(ref, got)
+    assert (got["collisionfree"] == 0).sum() >= 10
+    # the switch off: the rotation path, as a direct C2A_QueryTimeOfContact
+    rot = G.ccd_batch(meshes, G.make_queries(*P), G.default_params(translation=0))
+    O.set_solve_translation(0)
+    try:
+        rref = O.batch_toc(m1, m2, *P, 1e-4, 1e-4, seeds=np.zeros((64, 2), np.int32),
+                           with_contacts=True, nthreads=8)
+    finally:
+        O.set_solve_translation(1)
+    _same(rref, rot)
