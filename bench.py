@@ -40,8 +40,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nq", type=int, default=0, help="queries (default: the workload's)")
-    ap.add_argument("--workload", choices=["knot", "uneven", "grazing"], default="knot",
-                    help="knot: the headline batch; uneven / grazing: the other configurations")
+    ap.add_argument("--workload", choices=["knot", "uneven", "grazing", "demo"], default="knot",
+                    help="knot: the headline batch; uneven / grazing: the other configurations; "
+                         "demo: the reference demo's 303 bunny-vs-bunny queries")
     ap.add_argument("--spec-depth", type=int, default=0)
     ap.add_argument("--order", choices=["both", "plain", "learned"], default="both",
                     help="time the plain order, the learned order, or both (value = plain)")
@@ -149,6 +150,16 @@ def load_workload(name, nq):
                     desc="uneven batch: %d queries over icospheres of 80..81,920 triangles "
                          "(value noise 0.1), a 64x32 torus and the knot; log-uniform mesh "
                          "sizes, rigid motions spanning 0.5-4 body radii" % nq)
+    if name == "demo":
+        t9, idx = W.bunny()
+        P = W.demo_queries()
+        n = len(P[0])
+        return dict(tris=[t9, t9], vidx=[idx, idx], P=P,
+                    pairs=np.tile(np.array([0, 1], np.int32), (n, 1)), tol_d=1e-4, tol_t=1e-4,
+                    nq=n, sample=n,
+                    desc="reference demo: %d C2A_Solve queries, two 69,664-triangle bunnies "
+                         "(tri_models/bunny_noholes.tri) moved by the torus-knot animations, "
+                         "tol_d = tol_t = 1e-4" % n)
     cfg = W.CONFIG_GRAZING
     nq = nq or cfg["nq"]
     s, idx = W.uv_sphere(cfg["n_lon"], cfg["n_lat"], cfg["noise"], 3)

@@ -414,7 +414,7 @@ C2A_HD void pair_feature_row(uint16_t* out, node_ptr N1, node_ptr N2, int b1, in
   const double l1[2] = {A1->l[0], A1->l[1]}, l2[2] = {A2->l[0], A2->l[1]};
   float f[NF];
   pair_features(f, l1, l2, A1->r, A2->r, R, T, d, mt, mb[0], mb[1], A1->depth, A2->depth,
-                A1->angR, A2->angR, mo1, mo2, up, fl & 3);
+                A1->angR, A2->angR, mo1, mo2, up, fl & 3, 1.0 / N1->angR);
   for (int k = 0; k < NF; k++) out[k] = f32_to_bf16(f[k]);
 }
 
@@ -825,7 +825,7 @@ C2A_HD int lds_max(int32_t* p, int v) {
 // Wavefront-aggregated: one atomic per 64 records.
 template <class LeaderT>
 C2A_HD void write_samples(const LeaderT& L, const KArgs& a, const ArenaView& av, int nrec,
-                          int lane, int nl) {
+                          double inv_s, int lane, int nl) {
   const double C2A_GLOBAL* ml = gp(av.minleaf);
   const uint16_t C2A_GLOBAL* fe = gp(av.feat);
   for (int base = 1; base < nrec; base += nl) {
@@ -842,7 +842,7 @@ C2A_HD void write_samples(const LeaderT& L, const KArgs& a, const ArenaView& av,
     if (take && slot < a.sample_cap) {
       float C2A_GLOBAL* o = gp(a.samples) + slot * (NF + 1);
       for (int k = 0; k < NF; k++) o[k] = bf16_to_f32(fe[(int64_t)r * NF + k]);
-      o[NF] = slog(ml[r]);
+      o[NF] = slog(ml[r] * inv_s);
       gp(a.sample_keys)[slot] = key;
     }
   }
@@ -1081,7 +1081,7 @@ C2A_HD void replay_layout(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, 
     }
     mem_sync();
   }
-  if (sampling) write_samples(L, a, av, nrec, lane, nl);
+  if (sampling) write_samples(L, a, av, nrec, 1.0 / nodes_of(mesh_of(g, 0))->angR, lane, nl);
   if (lane == 0) pos[0] = -1;
   mem_sync();
   for (int v = 0; v < top; v++) {

@@ -53,36 +53,40 @@ C2A_HD float bf16_to_f32(uint16_t h) {
 
 C2A_HD float slog(double v) { return (float)log(v + 1e-4); }
 
-// The 32 features of a pair (frozen list):
+// The 32 features of a pair (frozen list).  Lengths are in units of the
+// query's scale s (the angular radius of mesh 1's root: the largest vertex
+// distance from its origin), so that meshes of any size look alike:
 //  0-1 side lengths of node 1's rectangle, 2-3 of node 2's, 4-5 radii,
 //  6-8 relative translation, 9-17 relative rotation (row-major),
 //  18 log RSS distance, 19 log admissible time, 20-21 log motion bounds,
 //  22-23 depths of the two nodes, 24-25 angular radii, 26 |cv| of object 1,
 //  27 rotation angle of object 1, 28 |cv| of object 2, 29 rotation angle of
 //  object 2, 30 remaining time (UpboundTOC), 31 leaf flags (node1 + 2 node2) / 3.
+// inv_s = 1 / s.  The training label is slog(closest distance * inv_s).
 C2A_HD void pair_features(float* f, const double* l1, const double* l2, double r1, double r2,
                           const double* R, const double* T, double d, double mt, double mb1,
                           double mb2, int depth1, int depth2, double angR1, double angR2,
-                          const Motion& mo1, const Motion& mo2, double up, int leafbits) {
-  f[0] = (float)l1[0];
-  f[1] = (float)l1[1];
-  f[2] = (float)l2[0];
-  f[3] = (float)l2[1];
-  f[4] = (float)r1;
-  f[5] = (float)r2;
-  for (int i = 0; i < 3; i++) f[6 + i] = (float)T[i];
+                          const Motion& mo1, const Motion& mo2, double up, int leafbits,
+                          double inv_s) {
+  f[0] = (float)(l1[0] * inv_s);
+  f[1] = (float)(l1[1] * inv_s);
+  f[2] = (float)(l2[0] * inv_s);
+  f[3] = (float)(l2[1] * inv_s);
+  f[4] = (float)(r1 * inv_s);
+  f[5] = (float)(r2 * inv_s);
+  for (int i = 0; i < 3; i++) f[6 + i] = (float)(T[i] * inv_s);
   for (int i = 0; i < 9; i++) f[9 + i] = (float)R[i];
-  f[18] = slog(d);
+  f[18] = slog(d * inv_s);
   f[19] = slog(mt);
-  f[20] = slog(mb1);
-  f[21] = slog(mb2);
+  f[20] = slog(mb1 * inv_s);
+  f[21] = slog(mb2 * inv_s);
   f[22] = (float)depth1 * 0.0625f;
   f[23] = (float)depth2 * 0.0625f;
-  f[24] = (float)angR1;
-  f[25] = (float)angR2;
-  f[26] = (float)sqrt(dot3(mo1.cv, mo1.cv));
+  f[24] = (float)(angR1 * inv_s);
+  f[25] = (float)(angR2 * inv_s);
+  f[26] = (float)(sqrt(dot3(mo1.cv, mo1.cv)) * inv_s);
   f[27] = (float)mo1.angVel;
-  f[28] = (float)sqrt(dot3(mo2.cv, mo2.cv));
+  f[28] = (float)(sqrt(dot3(mo2.cv, mo2.cv)) * inv_s);
   f[29] = (float)mo2.angVel;
   f[30] = (float)up;
   f[31] = (float)leafbits * (1.0f / 3.0f);

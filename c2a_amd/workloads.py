@@ -11,6 +11,7 @@ gives the same bytes on every machine.  The reference ships no knot mesh
 are this project's choices and are documented with their parameters.
 """
 import math
+import os
 
 import numpy as np
 
@@ -346,3 +347,44 @@ def grazing_queries(nq, seed, radius=1.0, max_sep=5e-3):
 CONFIG_UNEVEN = dict(name="uneven batched CCD", nq=16384, seed=20262, tol_d=1e-3, tol_t=1e-4)
 CONFIG_GRAZING = dict(name="large grazing-contact stress", nq=1024, seed=20264, tol_d=1e-3,
                       tol_t=1e-4, n_lon=512, n_lat=256, noise=0.1)
+
+
+# ---------------------------------------------------------------------------
+# The reference demo (CCDDemo/mainTorusknot.cpp): two copies of the Stanford
+# bunny (tri_models/bunny_noholes.tri, 69,664 triangles) moved by the two
+# torus-knot animation files (models/torusknot{1,2}.ani, 303 frames); the
+# data are fixtures written by tests/golden/make_fixtures.py.
+# ---------------------------------------------------------------------------
+_FIXTURES = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests",
+                         "golden")
+
+
+def bunny(path=None):
+    """(triangles (n, 9) float64, vertex indices (n, 3) int32) of the demo's
+    bunny, in the file's triangle order."""
+    d = np.load(path or os.path.join(_FIXTURES, "bunny_noholes.npz"))
+    v, f = d["vertices"], d["faces"]
+    return v[f].reshape(-1, 9).copy(), f.astype(np.int32)
+
+
+def demo_queries(path=None):
+    """The demo's 303 C2A_Solve queries (CCDDemo/mainTorusknot.cpp:216-266):
+    object 1 moves from frame step1 to step2 of animation 1 ((0, 1) for
+    frames below 101, (101, 102) below 202, (202, 203) after); object 2
+    keeps the rotation of frame `iframe` of animation 2 and moves between
+    the translations of frames step1 and step2.  Returns (P00, P01, P10, P11)."""
+    d = np.load(path or os.path.join(_FIXTURES, "torusknot_ani.npz"))
+    R, T = d["R"], d["T"]
+    n = R.shape[1]
+    P = [np.zeros((n, 12)) for _ in range(4)]
+    for f in range(n):
+        s1, s2 = (0, 1) if f < 101 else ((101, 102) if f < 202 else (202, 203))
+        P[0][f] = np.concatenate([R[0][s1].reshape(9), T[0][s1]])
+        P[1][f] = np.concatenate([R[0][s2].reshape(9), T[0][s2]])
+        P[2][f] = np.concatenate([R[1][f].reshape(9), T[1][s1]])
+        P[3][f] = np.concatenate([R[1][f].reshape(9), T[1][s2]])
+    return P
+
+
+CONFIG_DEMO = dict(name="reference demo: bunny vs bunny, torus-knot animation", nq=303,
+                   tol_d=1e-4, tol_t=1e-4)

@@ -107,3 +107,41 @@ def test_solve_demo_matches_oracle(tmp_path):
             assert g["i5"][1] == res.num_contact, i
             assert np.array_equal(g["tr"][:9], R0.reshape(-1)) and np.array_equal(g["tr"][9:12], T0), i
             assert np.array_equal(g["tr"][12:21], R1.reshape(-1)) and np.array_equal(g["tr"][21:], T1), i
+
+
+@pytest.mark.gpu
+def test_reference_demo_bunny_matches_oracle(tmp_path):
+    """The reference's own demo data: two 69,664-triangle bunnies
+    (tri_models/bunny_noholes.tri) moved by the torus-knot animations
+    (models/torusknot{1,2}.ani), the demo's 303 C2A_Solve calls in order with
+    its last_tri protocol (CCDDemo/mainTorusknot.cpp:216-321)."""
+    t9, _ = W.bunny()
+    t9 = np.ascontiguousarray(t9)
+    P = W.demo_queries()
+    nq = len(P[0])
+    poses = np.stack(P, axis=1)
+    payload = (_i32(len(t9)) + t9.tobytes() + _i32(len(t9)) + t9.tobytes() + _i32(nq) +
+               np.ascontiguousarray(poses).tobytes())
+    print("solve_demo on the bunny demo: %d queries" % nq, flush=True)
+    b = _run("solve_demo", payload, tmp_path)
+    rec = np.dtype([("td", "<f8", 2), ("i5", "<i4", 5), ("tr", "<f8", 24)])
+    got = np.frombuffer(b, rec)
+    assert len(got) == nq
+    m1, m2 = O.OModel(t9), O.OModel(t9)
+    hits = 0
+    for i in range(nq):
+        p = poses[i]
+        res, (R0, T0), (R1, T1) = O.solve(m1, m2, (p[0, :9].reshape(3, 3), p[0, 9:]),
+                                          (p[1, :9].reshape(3, 3), p[1, 9:]),
+                                          (p[2, :9].reshape(3, 3), p[2, 9:]),
+                                          (p[3, :9].reshape(3, 3), p[3, 9:]))
+        g = got[i]
+        assert g["td"][0] == res.toc and g["td"][1] == res.distance, i
+        assert g["i5"][0] == res.numCA and g["i5"][2] == res.collisionfree, i
+        assert g["i5"][3] == m1.m.last_tri and g["i5"][4] == m2.m.last_tri, i
+        if not res.collisionfree:
+            hits += 1
+            assert g["i5"][1] == res.num_contact, i
+            assert np.array_equal(g["tr"][:9], R0.reshape(-1)) and np.array_equal(g["tr"][9:12], T0), i
+            assert np.array_equal(g["tr"][12:21], R1.reshape(-1)) and np.array_equal(g["tr"][21:], T1), i
+    assert hits > 100

@@ -62,3 +62,18 @@ def test_grazing_262k_bit_exact():
     ref = O.batch_toc(m1, m2, *P, cfg["tol_d"], cfg["tol_t"], seeds=np.zeros((nq, 2), np.int32),
                       with_contacts=True, nthreads=8)
     check(ref, got, "grazing")
+
+
+def test_reference_demo_batch_bit_exact():
+    """The reference demo's 303 bunny-vs-bunny queries as one batch
+    (independent queries, C2A_Solve's tolerances and contact count)."""
+    t9, idx = W.bunny()
+    P = W.demo_queries()
+    nq = len(P[0])
+    print("demo batch nq=%d, 69,664 triangles per bunny" % nq, flush=True)
+    meshes = [G.Mesh(t9, vidx=idx), G.Mesh(t9, vidx=idx)]
+    got = G.ccd_batch(meshes, G.make_queries(*P), G.default_params())
+    ref = O.batch_toc(O.OModel(t9), O.OModel(t9), *P, 1e-4, 1e-4,
+                      seeds=np.zeros((nq, 2), np.int32), with_contacts=True, nthreads=8)
+    check(ref, got, "demo")
+    assert (got["collisionfree"] == 0).sum() > 100

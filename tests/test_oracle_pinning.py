@@ -426,3 +426,20 @@ def test_solve_reference_semantics(spheres):
     mo1 = O.OMotion(I, np.zeros(3), Rz, np.zeros(3))
     assert np.allclose(R0o, mo1.integrate(res.toc)[0], atol=1e-15)
     assert (m1.last_tri, m2.last_tri) == (res.out_last_triA, res.out_last_triB)
+
+
+def test_demo_fixtures_are_the_reference_data():
+    """The committed demo fixtures (tests/golden/make_fixtures.py): the
+    bunny's counts as the .tri header states them, a closed mesh (every edge
+    shared by two triangles), and rotation matrices in every animation frame."""
+    from c2a_amd import workloads as W
+    t9, f = W.bunny()
+    assert t9.shape == (69664, 9) and f.max() == 34833
+    e = np.sort(np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), axis=1)
+    _, cnt = np.unique(e, axis=0, return_counts=True)
+    assert (cnt == 2).all()
+    P = W.demo_queries()
+    assert P[0].shape == (303, 12)
+    for p in P:
+        R = p[:, :9].reshape(-1, 3, 3)
+        assert np.allclose(R @ R.transpose(0, 2, 1), np.eye(3), atol=2e-5)
