@@ -621,6 +621,63 @@ int c2a_contact_count(c2a_mesh* const* meshes, int32_t nmesh, const double* pose
   return C2A_OK;
 }
 
+int c2a_distance(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                 const int32_t* mesh_pairs, const int32_t* seed_tris, double rel_err,
+                 double abs_err, int64_t nq, c2a_distance_result* out, void* stream) {
+  if (nq < 0 || (nq > 0 && (!poses || !mesh_pairs || !out)))
+    return fail(C2A_ERR_BAD_ARG, "bad arrays");
+  if (!(rel_err >= 0) || !(abs_err >= 0))
+    return fail(C2A_ERR_BAD_ARG, "error bounds must be >= 0");
+  if (nq == 0) return C2A_OK;
+  int rc = check_device();
+  if (rc) return rc;
+  std::vector<MeshDev> tab;
+  if ((rc = make_mesh_table(meshes, nmesh, tab))) return rc;
+  std::vector<ContactDev> hc((size_t)nq);
+  for (int64_t i = 0; i < nq; i++) {
+    ContactDev& c = hc[(size_t)i];
+    memset(&c, 0, sizeof(c));
+    for (int k = 0; k < 2; k++) {
+      int m = mesh_pairs[2 * i + k];
+      if (m < 0 || m >= nmesh) return fail(C2A_ERR_BAD_ARG, "mesh index out of range");
+      c.mesh[k] = m;
+      int t = seed_tris ? seed_tris[2 * i + k] : 0;
+      if (t < 0) t = 0;
+      if (t >= tab[(size_t)m].ntri) return fail(C2A_ERR_BAD_ARG, "seed triangle out of range");
+      c.seed[k] = t;
+    }
+    unpack_pose(poses + 24 * i, c.R1, c.T1);
+    unpack_pose(poses + 24 * i + 12, c.R2, c.T2);
+    c.rel_err = rel_err;
+    c.abs_err = abs_err;
+  }
+  int CAP = pick_cap(tab, mesh_pairs, nq, 2);
+  if (CAP < 0) return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
+  const int K = 5;
+  DevBuf dm, din, dout, dc;
+  if ((rc = dm.alloc(tab.size() * sizeof(MeshDev))) ||
+      (rc = din.alloc((size_t)nq * sizeof(ContactDev))) ||
+      (rc = dout.alloc((size_t)nq * sizeof(c2a_distance_result))) ||
+      (rc = dc.alloc(sizeof(int64_t))))
+    return rc;
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpy(dm.p, tab.data(), tab.size() * sizeof(MeshDev), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(din.p, hc.data(), (size_t)nq * sizeof(ContactDev), hipMemcpyHostToDevice));
+  KArgs a;
+  memset(&a, 0, sizeof(a));
+  a.meshes = (const MeshDev*)dm.p;
+  a.contact_in = (const ContactDev*)din.p;
+  a.dist_out = (c2a_distance_result*)dout.p;
+  a.counter = (int64_t*)dc.p;
+  a.nq = nq;
+  a.mode = MODE_DISTANCE;
+  if ((rc = run_kernel(a, K, CAP, 1, grid_blocks(K, CAP, 1, nq), s))) return rc;
+  HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)nq * sizeof(c2a_distance_result),
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return C2A_OK;
+}
+
 int c2a_motion_prepare(const double* pose_start, const double* pose_end, double* motion24) {
   if (!pose_start || !pose_end || !motion24) return fail(C2A_ERR_BAD_ARG, "null argument");
   double R0[9], T0[3], R1[9], T1[3];

@@ -1343,7 +1343,10 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
     const int flags = c.flags;
     if (!(flags & 4)) {
       bool desc;
-      if (contact) desc = c.d < L.thr;   // abs = rel = 0 (C2A.cpp:1760-1761)
+      if (contact)
+        desc = (L.mode == MODE_DISTANCE)   // C2A_PQP.cpp:594-617
+                   ? ((c.d < (L.D - L.abs_err)) || (c.d * (1 + L.rel_err) < L.D))
+                   : c.d < L.thr;   // abs = rel = 0 (C2A.cpp:1760-1761)
       else desc = c.mt < L.Up && ((c.d < (L.D - L.abs_err)) || (c.d * (1 + L.rel_err) < L.D));
       if (!desc) {
         if (!contact && c.mt < L.M) L.M = c.mt;
@@ -1386,6 +1389,15 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
           L.last2 = uni(sl.t2);
         }
         L.tri++;
+      } else if (L.mode == MODE_DISTANCE) {   // C2A_PQP.cpp:493-520 (strict <)
+        const double dTri = unid(sl.dTri);
+        if (dTri < L.D) {
+          L.D = dTri;
+          for (int i = 0; i < 3; i++) { L.p1[i] = unid(sl.p[i]); L.p2[i] = unid(sl.q[i]); }
+          L.last1 = uni(sl.t1);
+          L.last2 = uni(sl.t2);
+        }
+        L.tri++;
       } else {
         if ((uni(sl.flags) & 4) && unid(sl.dTri) <= L.thr) L.num_contact++;
       }
@@ -1404,7 +1416,7 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
       L.sp = 0;
       return false;
     }
-    if (!contact) L.bv += 2;
+    if (!contact || L.mode == MODE_DISTANCE) L.bv += 2;
     // log the expansion for the next step's replay
     int nnA = -1;
     if (c.nn >= 0 && L.nlog < a.logcap) {
@@ -1500,15 +1512,17 @@ template <int K, int CAP>
 C2A_HD void begin_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   const MeshDev M1 = mesh_of(g, 0);
   const MeshDev M2 = mesh_of(g, 1);
-  double R1[9], T1[3], R2[9], T2[3];
-  if (L.numCA == 0) {
-    for (int i = 0; i < 9; i++) { R1[i] = g.mo[0].R0[i]; R2[i] = g.mo[1].R0[i]; }
-    for (int i = 0; i < 3; i++) { T1[i] = g.mo[0].T0[i]; T2[i] = g.mo[1].T0[i]; }
-  } else {
-    motion_integrate(g.mo[0], L.lamda, R1, T1, nullptr);
-    motion_integrate(g.mo[1], L.lamda, R2, T2, nullptr);
+  if (L.mode != MODE_DISTANCE) {   // a distance query's traversal starts in fetch
+    double R1[9], T1[3], R2[9], T2[3];
+    if (L.numCA == 0) {
+      for (int i = 0; i < 9; i++) { R1[i] = g.mo[0].R0[i]; R2[i] = g.mo[1].R0[i]; }
+      for (int i = 0; i < 3; i++) { T1[i] = g.mo[0].T0[i]; T2[i] = g.mo[1].T0[i]; }
+    } else {
+      motion_integrate(g.mo[0], L.lamda, R1, T1, nullptr);
+      motion_integrate(g.mo[1], L.lamda, R2, T2, nullptr);
+    }
+    start_traversal(g, L, M1, M2, R1, T1, R2, T2, a.logcap > 0 ? 0 : -1, lane);
   }
-  start_traversal(g, L, M1, M2, R1, T1, R2, T2, a.logcap > 0 ? 0 : -1, lane);
   L.nlog = 0;
   L.nflush = 0;
   L.pq_pending = 0;
@@ -1523,6 +1537,11 @@ C2A_HD void begin_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   for (int i = 0; i < 9; i++) Rrel[i] = g.Rrel[i];
   for (int i = 0; i < 3; i++) Trel[i] = g.Trel[i];
   L.D = tri_distance(Rrel, Trel, ta, tb, p, q, nullptr);
+  if (L.mode == MODE_DISTANCE) {   // C2A_Distance's first bound (C2A_PQP.cpp:998-1005)
+    for (int i = 0; i < 3; i++) { L.p1[i] = p[i]; L.p2[i] = q[i]; }
+    L.state = ST_CONTACT_WALK;
+    return;
+  }
   if (L.numCA == 0) L.M = 1;
   // controlled error schedule (C2A.cpp:1869-1891)
   if (L.M <= 0.005 || L.D <= 0.5 || L.numCA > 5) {
@@ -1716,7 +1735,9 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     L.state = ST_WALK;
     return true;
   }
-  if (a.mode == MODE_CONTACT) {
+  if (a.mode == MODE_CONTACT || a.mode == MODE_DISTANCE) {
+    // contact enumeration, or C2A_Distance with qsize <= 2 (C2A_PQP.cpp:970-1057):
+    // both walk pair distances without motion bounds
     const ContactDev& c = a.contact_in[qi];
     if (lane == 0) {
       g.mesh[0] = c.mesh[0];
@@ -1726,16 +1747,23 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
       g.contact = 1;
     }
     lane_sync();
-    L.mode = MODE_CONTACT;
+    L.mode = a.mode;
     L.thr = c.threshold;
     L.num_contact = 0;
+    L.rel_err = c.rel_err;
+    L.abs_err = c.abs_err;
+    L.seed1 = c.seed[0];
+    L.seed2 = c.seed[1];
+    L.last1 = c.seed[0];
+    L.last2 = c.seed[1];
     const MeshDev M1 = a.meshes[c.mesh[0]];
     const MeshDev M2 = a.meshes[c.mesh[1]];
     double R1[9], T1[3], R2[9], T2[3];
     for (int i = 0; i < 9; i++) { R1[i] = c.R1[i]; R2[i] = c.R2[i]; }
     for (int i = 0; i < 3; i++) { T1[i] = c.T1[i]; T2[i] = c.T2[i]; }
     start_traversal(g, L, M1, M2, R1, T1, R2, T2, -1, lane);
-    L.state = ST_CONTACT_WALK;
+    // a distance query takes its first bound in begin_step
+    L.state = a.mode == MODE_DISTANCE ? ST_STEP : ST_CONTACT_WALK;
     return true;
   }
   const QueryDev& q = a.q[qi];
@@ -1907,6 +1935,7 @@ C2A_HD void leader_run(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int
         break;
       case ST_STEP:
         begin_step(g, L, a, lane);
+        if (L.state == ST_CONTACT_WALK) break;   // distance query
         if (a.logcap > 0 && L.nlog_prev > 0) {
           replay(g, L, a, lane, nl);
           L.state = ST_LAYOUT;
@@ -1952,7 +1981,24 @@ C2A_HD void leader_run(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int
       case ST_CONTACT_WALK:
         if (walk(g, L, a, lane, nl)) return;
         if (lane == 0) {
-          if (L.mode == MODE_CONTACT) a.counts_out[L.qi] = L.num_contact;
+          if (L.mode == MODE_DISTANCE) {
+            // p2 back into model 2's frame (C2A_PQP.cpp:1043-1047)
+            c2a_distance_result* r = a.dist_out + L.qi;
+            double u[3], Rr[9], Tr[3], q2[3];
+            for (int i = 0; i < 9; i++) Rr[i] = g.Rrel[i];
+            for (int i = 0; i < 3; i++) Tr[i] = g.Trel[i];
+            sub3(u, L.p2, Tr);
+            mtxv(q2, Rr, u);
+            r->distance = L.D;
+            for (int i = 0; i < 3; i++) { r->p1[i] = L.p1[i]; r->p2[i] = q2[i]; }
+            r->bv_tests = L.bv;
+            r->tri_tests = L.tri;
+            r->last_tri[0] = L.last1;
+            r->last_tri[1] = L.last2;
+            r->flags = L.flags;
+            r->pad = 0;
+          } else if (L.mode == MODE_CONTACT)
+            a.counts_out[L.qi] = L.num_contact;
           else a.out[L.qi].num_contact = L.num_contact;
         }
         L.state = ST_FETCH;

@@ -12,7 +12,7 @@
 namespace c2a {
 
 
-enum : int { MODE_QUERY = 0, MODE_STEP = 1, MODE_CONTACT = 2 };
+enum : int { MODE_QUERY = 0, MODE_STEP = 1, MODE_CONTACT = 2, MODE_DISTANCE = 3 };
 
 struct MeshDev {
   const GpuNode* nodes;
@@ -40,11 +40,14 @@ struct StepDev {
   int32_t pad;
 };
 
-// One contact enumeration at explicit poses (backs C2A_QueryContact).
+// One contact enumeration at explicit poses (backs C2A_QueryContact), or
+// one static distance query (backs C2A_Distance: seed pair, error bounds).
 struct ContactDev {
   double R1[9], T1[3], R2[9], T2[3];
   double threshold;
   int32_t mesh[2];
+  int32_t seed[2];
+  double rel_err, abs_err;
 };
 
 // Replay of the previous CA step.  Consecutive steps of one query visit
@@ -157,6 +160,7 @@ struct KArgs {
   c2a_step_result* step_out;
   const ContactDev* contact_in;
   int32_t* counts_out;
+  c2a_distance_result* dist_out;
   int64_t* counter;        // work-queue ticket, zeroed before each launch
   int64_t nq;
   double tol_d, tol_t;

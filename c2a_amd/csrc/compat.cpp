@@ -3,6 +3,7 @@
 // engine motion records, and each query is a one-query batch on the GPU.
 #include <string.h>
 
+#include <chrono>
 #include <vector>
 
 #include "../../include/c2a/C2A.h"
@@ -336,6 +337,46 @@ int C2A_TimeOfContactStep(CInterpMotion* m1, CInterpMotion* m2, C2A_TimeOfContac
   res->num_tri_tests += (int)r.tri_tests;
   set_last(o1, r.last_tri[0]);
   set_last(o2, r.last_tri[1]);
+  return PQP_OK;
+}
+
+int C2A_Distance(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
+                 PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,
+                 PQP_REAL abs_err, int qsize) {
+  if (o1->build_state != 2 || o2->build_state != 2) return PQP_ERR_UNPROCESSED_MODEL;
+  const auto t0 = std::chrono::steady_clock::now();
+  c2a_mesh* meshes[2] = {o1->engine_mesh(), o2->engine_mesh()};
+  double poses[24];
+  pose12_from(R1, T1, poses);
+  pose12_from(R2, T2, poses + 12);
+  const int32_t pairs[2] = {0, 1};
+  const int32_t seeds[2] = {o1->tri_index(o1->last_tri), o2->tri_index(o2->last_tri)};
+  c2a_distance_result r;
+  if (c2a_distance(meshes, 2, poses, pairs, seeds, rel_err, abs_err, 1, &r, nullptr) != C2A_OK)
+    return -1;
+  // [R, T] = [R1, T1]^T [R2, T2] (C2A_PQP.cpp:990-993)
+  for (int i = 0; i < 3; i++) {
+    for (int j = 0; j < 3; j++)
+      res->R[i][j] = R1[0][i] * R2[0][j] + R1[1][i] * R2[1][j] + R1[2][i] * R2[2][j];
+    res->T[i] = R1[0][i] * (T2[0] - T1[0]) + R1[1][i] * (T2[1] - T1[1]) +
+                R1[2][i] * (T2[2] - T1[2]);
+  }
+  res->rel_err = rel_err;
+  res->abs_err = abs_err;
+  res->distance = r.distance;
+  for (int i = 0; i < 3; i++) {
+    res->p1[i] = r.p1[i];
+    res->p2[i] = r.p2[i];
+  }
+  res->qsize = qsize;
+  res->num_bv_tests = (int)r.bv_tests;
+  res->num_tri_tests = (int)r.tri_tests;
+  o1->last_tri = &o1->tris[r.last_tri[0]];
+  o2->last_tri = &o2->tris[r.last_tri[1]];
+  res->t1 = o1->last_tri->id;
+  res->t2 = o2->last_tri->id;
+  res->query_time_secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return PQP_OK;
 }
 

@@ -80,6 +80,44 @@ int C2A_TimeOfContactStep(CInterpMotion* objmotion1, CInterpMotion* objmotion2,
                           C2A_Model* o1, PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2,
                           PQP_REAL tolerance_t, PQP_REAL tolerance_d);
 
+// Result of the static distance query (C2A_Internal.h:89-93 over PQP's
+// PQP_DistanceResult): R, T take model 2 into model 1's frame; p1 is in
+// model 1's frame, p2 in model 2's; t1, t2 are the caller ids of the
+// closest pair.
+struct C2A_DistanceResult {
+  PQP_REAL R[3][3];
+  PQP_REAL T[3];
+  PQP_REAL rel_err;
+  PQP_REAL abs_err;
+  PQP_REAL distance;
+  PQP_REAL p1[3];
+  PQP_REAL p2[3];
+  int qsize;
+  int num_bv_tests;
+  int num_tri_tests;
+  double query_time_secs;
+  int t1;
+  int t2;
+
+  int NumBVTests() { return num_bv_tests; }
+  int NumTriTests() { return num_tri_tests; }
+  double QueryTimeSecs() { return query_time_secs; }
+  PQP_REAL Distance() { return distance; }
+  const PQP_REAL* P1() { return p1; }
+  const PQP_REAL* P2() { return p2; }
+};
+
+// Closest triangle pair of two models at the given poses, within the
+// relative and absolute error bounds (C2A.h:255-259, C2A_PQP.cpp:970-1057).
+// The first bound is the models' last_tri pair; both are updated.  qsize > 2
+// (the reference's priority-queue search) runs the same depth-first search,
+// which meets the same error bounds; its counters then differ from the
+// reference's.  Returns PQP_OK, PQP_ERR_UNPROCESSED_MODEL, or -1 when the
+// engine reports an error.
+int C2A_Distance(C2A_DistanceResult* result, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
+                 PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,
+                 PQP_REAL abs_err, int qsize = 2);
+
 // Counts the triangle pairs within `threshold` at the motions' current poses.
 PQP_REAL C2A_QueryContact(CInterpMotion* objmotion1, CInterpMotion* objmotion2,
                           C2A_TimeOfContactResult* res, C2A_Model* o1, C2A_Model* o2,

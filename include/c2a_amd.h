@@ -201,6 +201,26 @@ int c2a_contact_count(c2a_mesh *const *meshes, int32_t nmesh, const double *pose
                       const int32_t *mesh_pairs /* nq x 2 */, const double *thresholds, int64_t nq,
                       int32_t *counts_out, void *stream);
 
+/* Static distance query at explicit poses (C2A_Distance with qsize <= 2,
+ * C2A/src/C2A_PQP.cpp:970-1057 -> C2ADistanceRecurse :481-620): the closest
+ * triangle pair within the relative / absolute error bounds, searched depth
+ * first, closer pair first, from the bound of the seed pair.  seed_tris (nq x
+ * 2, or NULL for triangle 0) plays the models' last_tri; the result's
+ * last_tri is the models' last_tri after the query (build order; caller ids
+ * through c2a_mesh_tri_ids).  p1 is in model 1's frame, p2 in model 2's. */
+typedef struct {
+  double distance;
+  double p1[3], p2[3];
+  int64_t bv_tests, tri_tests;
+  int32_t last_tri[2];
+  int32_t flags;        /* C2A_FLAG_STACK_OVERFLOW: the search was cut short */
+  int32_t pad;
+} c2a_distance_result;
+int c2a_distance(c2a_mesh *const *meshes, int32_t nmesh, const double *poses /* nq x 2 x 12 */,
+                 const int32_t *mesh_pairs /* nq x 2 */, const int32_t *seed_tris /* nq x 2 */,
+                 double rel_err, double abs_err, int64_t nq, c2a_distance_result *out,
+                 void *stream);
+
 /* Linear motion between two poses, as the kernel integrates it
  * (CInterpMotion_Linear ctor + velocity(), InterpMotion.cpp:148-270,
  * 486-513): motion24 = R0[9], T0[3], q0[4] (x,y,z,w), cv[3], axis[3],

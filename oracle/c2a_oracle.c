@@ -1654,6 +1654,93 @@ int c2o_query_contact(const c2o_model *m1, const c2o_model *m2,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Static distance query (C2A_PQP.cpp:970-1057 -> 481-620, qsize <= 2)       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  c2o_model *m1, *m2;
+  m3 Rrel;
+  double Trel[3];
+  double rel_err, abs_err;
+  c2o_distance_result *r;
+} distance_ctx;
+
+/* C2ADistanceRecurse (C2A_PQP.cpp:481-620): closer child pair first, a pair
+ * is entered while d < D - abs_err or d (1 + rel_err) < D; leaves replace the
+ * bound only when strictly closer. */
+static void distance_recurse(distance_ctx *cx, const m3 R, const double T[3], int b1, int b2) {
+  const c2o_bv *B1 = &cx->m1->bvs[b1], *B2 = &cx->m2->bvs[b2];
+  int l1 = B1->first_child < 0, l2 = B2->first_child < 0;
+  c2o_distance_result *r = cx->r;
+  if (l1 && l2) {
+    double p[3], q[3];
+    int t1 = -B1->first_child - 1, t2 = -B2->first_child - 1;
+    int ta, fa, tb, fb;
+    r->tri_tests++;
+    double d = tri_distance(cx->Rrel, cx->Trel, &cx->m1->tris[t1], &cx->m2->tris[t2], p, q,
+                            &ta, &fa, &tb, &fb);
+    if (d < r->distance) {
+      r->distance = d;
+      memcpy(r->p1, p, sizeof(p));
+      memcpy(r->p2, q, sizeof(q));   /* model 1's frame until the end */
+      cx->m1->last_tri = t1;
+      cx->m2->last_tri = t2;
+    }
+    return;
+  }
+  int a1, a2, c1, c2;
+  m3 R1, R2;
+  double T1[3], T2[3], Tt[3], S[3];
+  if (l2 || (!l1 && (B1->size > B2->size))) {
+    a1 = B1->first_child; c1 = a1 + 1; a2 = c2 = b2;
+    const c2o_bv *A = &cx->m1->bvs[a1], *C = &cx->m1->bvs[c1];
+    mtxm(R1, A->R, R); sub3(Tt, T, A->Tr); mtxv(T1, A->R, Tt);
+    mtxm(R2, C->R, R); sub3(Tt, T, C->Tr); mtxv(T2, C->R, Tt);
+  } else {
+    a1 = c1 = b1; a2 = B2->first_child; c2 = a2 + 1;
+    const c2o_bv *A = &cx->m2->bvs[a2], *C = &cx->m2->bvs[c2];
+    mxm(R1, R, A->R); mxvpv(T1, R, A->Tr, T);
+    mxm(R2, R, C->R); mxvpv(T2, R, C->Tr, T);
+  }
+  r->bv_tests += 2;
+  double d1 = bv_distance(R1, T1, &cx->m1->bvs[a1], &cx->m2->bvs[a2], S);
+  double d2 = bv_distance(R2, T2, &cx->m1->bvs[c1], &cx->m2->bvs[c2], S);
+#define ENTER(d) (((d) < (r->distance - cx->abs_err)) || ((d) * (1 + cx->rel_err) < r->distance))
+  if (d2 < d1) {
+    if (ENTER(d2)) distance_recurse(cx, R2, T2, c1, c2);
+    if (ENTER(d1)) distance_recurse(cx, R1, T1, a1, a2);
+  } else {
+    if (ENTER(d1)) distance_recurse(cx, R1, T1, a1, a2);
+    if (ENTER(d2)) distance_recurse(cx, R2, T2, c1, c2);
+  }
+#undef ENTER
+}
+
+void c2o_distance(c2o_model *m1, c2o_model *m2, const double R1[3][3], const double T1[3],
+                  const double R2[3][3], const double T2[3], double rel_err, double abs_err,
+                  c2o_distance_result *r) {
+  distance_ctx cx;
+  m3 R;
+  double T[3], p[3], q[3];
+  int ta, fa, tb, fb;
+  cx.m1 = m1; cx.m2 = m2; cx.rel_err = rel_err; cx.abs_err = abs_err; cx.r = r;
+  root_transform(m1, m2, R1, T1, R2, T2, cx.Rrel, cx.Trel, R, T);
+  /* first bound: the models' last_tri (C2A_PQP.cpp:998-1005) */
+  r->distance = tri_distance(cx.Rrel, cx.Trel, &m1->tris[m1->last_tri], &m2->tris[m2->last_tri],
+                             p, q, &ta, &fa, &tb, &fb);
+  memcpy(r->p1, p, sizeof(p));
+  memcpy(r->p2, q, sizeof(q));
+  r->bv_tests = 0;
+  r->tri_tests = 0;
+  distance_recurse(&cx, R, T, 0, 0);
+  /* p2 into model 2's frame (C2A_PQP.cpp:1043-1047) */
+  double u[3];
+  sub3(u, r->p2, cx.Trel);
+  mtxv(r->p2, cx.Rrel, u);
+  r->last_tri[0] = m1->last_tri;
+  r->last_tri[1] = m2->last_tri;
+}
+
+/* ------------------------------------------------------------------------ */
 /* C2A_Solve (C2A.cpp:2315-2444)                                             */
 /* ------------------------------------------------------------------------ */
 int c2o_solve(const double R00[3][3], const double T00[3],

@@ -23,6 +23,9 @@
  *   toc step / CA loop pinned by brute-force distance in exact mode,
  *                      scripted state machine, dense-sampling conservativeness
  *   contact count      pinned (brute-force count of pairs within threshold)
+ *   static distance    pinned (brute-force minimum over all triangle pairs at
+ *                      zero error bounds; error-bound envelope; closest
+ *                      points at the reported distance in both frames)
  */
 #ifndef C2A_ORACLE_H
 #define C2A_ORACLE_H
@@ -162,6 +165,19 @@ int c2o_query_contact(const c2o_model *m1, const c2o_model *m2,
                       const double R2[3][3], const double T2[3],
                       double threshold, c2o_contact *out, int cap,
                       long long *bv_tests);
+
+/* Static distance query (C2A_Distance with qsize <= 2, C2A_PQP.cpp:970-1057).
+ * The first bound is the pair (m1->last_tri, m2->last_tri); both are updated
+ * like the reference's.  p1 in model 1's frame, p2 in model 2's. */
+typedef struct {
+  double distance;
+  double p1[3], p2[3];
+  long long bv_tests, tri_tests;
+  int last_tri[2];
+} c2o_distance_result;
+void c2o_distance(c2o_model *m1, c2o_model *m2, const double R1[3][3], const double T1[3],
+                  const double R2[3][3], const double T2[3], double rel_err, double abs_err,
+                  c2o_distance_result *r);
 
 /* C2A_Solve (C2A.cpp:2315-2444): tol_d = tol_t = 1e-4, contact threshold
  * 2d+0.001.  Seeds from m*->last_tri like the demo (mainTorusknot.cpp:314). */

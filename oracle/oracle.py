@@ -99,6 +99,9 @@ def lib():
         L.c2o_query_contact.argtypes = [C.POINTER(Model), C.POINTER(Model), dp, dp, dp, dp,
                                         C.c_double, C.POINTER(Contact), C.c_int,
                                         C.POINTER(C.c_longlong)]
+        L.c2o_distance.restype = None
+        L.c2o_distance.argtypes = [C.POINTER(Model), C.POINTER(Model), dp, dp, dp, dp,
+                                   C.c_double, C.c_double, C.POINTER(DistanceResult)]
         L.c2o_toc_step.restype = None
         L.c2o_toc_step.argtypes = [C.POINTER(Model), C.POINTER(Model), C.POINTER(Motion),
                                    C.POINTER(Motion), dp, dp, dp, dp, C.POINTER(Model),
@@ -286,6 +289,23 @@ def query_contact(m1, m2, R1, T1, R2, T2, threshold, cap=0):
                                 _dp(f64(R2)), _dp(f64(T2)), float(threshold), out, int(cap),
                                 C.byref(bvt))
     return n, [out[i] for i in range(min(n, cap))], bvt.value
+
+
+class DistanceResult(C.Structure):
+    _fields_ = [("distance", C.c_double), ("p1", C.c_double * 3), ("p2", C.c_double * 3),
+                ("bv_tests", C.c_longlong), ("tri_tests", C.c_longlong),
+                ("last_tri", C.c_int * 2)]
+
+
+def distance(m1, m2, R1, T1, R2, T2, rel_err=0.0, abs_err=0.0, seeds=None):
+    """C2A_Distance (qsize <= 2) at explicit poses.  seeds sets the models'
+    last_tri first (build order); both are updated like the reference's."""
+    if seeds is not None:
+        m1.last_tri, m2.last_tri = int(seeds[0]), int(seeds[1])
+    r = DistanceResult()
+    lib().c2o_distance(C.byref(m1.m), C.byref(m2.m), _dp(f64(R1)), _dp(f64(T1)), _dp(f64(R2)),
+                       _dp(f64(T2)), float(rel_err), float(abs_err), C.byref(r))
+    return r
 
 
 def toc_step(m1, m2, mo1, mo2, R1, T1, R2, T2, seeds, numCA, mint_prev, upbound):

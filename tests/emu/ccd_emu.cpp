@@ -171,3 +171,41 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
     default: return small ? run<6, 48>(a) : run<6, 96>(a);
   }
 }
+
+// Static distance queries (MODE_DISTANCE) between two meshes, one lane.
+extern "C" int emu_distance(const double* tris1, int64_t n1, const double* tris2, int64_t n2,
+                            const double* poses, const int32_t* seeds, double rel_err,
+                            double abs_err, int64_t nq, c2a_distance_result* out) {
+  Bvh b1, b2;
+  if (build_bvh(tris1, nullptr, nullptr, n1, &b1) || build_bvh(tris2, nullptr, nullptr, n2, &b2))
+    return 1;
+  MeshDev tab[2];
+  tab[0].nodes = b1.nodes.data(); tab[0].tris = b1.tris.data();
+  tab[0].ntri = (int32_t)n1; tab[0].depth = b1.max_depth;
+  tab[1].nodes = b2.nodes.data(); tab[1].tris = b2.tris.data();
+  tab[1].ntri = (int32_t)n2; tab[1].depth = b2.max_depth;
+  std::vector<ContactDev> hc((size_t)nq);
+  for (int64_t i = 0; i < nq; i++) {
+    ContactDev& c = hc[(size_t)i];
+    memset(&c, 0, sizeof(c));
+    c.mesh[0] = 0;
+    c.mesh[1] = 1;
+    c.seed[0] = seeds ? seeds[2 * i] : 0;
+    c.seed[1] = seeds ? seeds[2 * i + 1] : 0;
+    memcpy(c.R1, poses + 24 * i, 72); memcpy(c.T1, poses + 24 * i + 9, 24);
+    memcpy(c.R2, poses + 24 * i + 12, 72); memcpy(c.T2, poses + 24 * i + 21, 24);
+    c.rel_err = rel_err;
+    c.abs_err = abs_err;
+  }
+  int64_t counter = 0;
+  KArgs a;
+  memset(&a, 0, sizeof(a));
+  a.meshes = tab;
+  a.contact_in = hc.data();
+  a.dist_out = out;
+  a.counter = &counter;
+  a.nq = nq;
+  a.mode = MODE_DISTANCE;
+  const bool small = b1.max_depth + b2.max_depth + 2 <= 48;
+  return small ? run<5, 48>(a) : run<5, 96>(a);
+}
