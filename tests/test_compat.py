@@ -145,3 +145,37 @@ def test_reference_demo_bunny_matches_oracle(tmp_path):
             assert np.array_equal(g["tr"][:9], R0.reshape(-1)) and np.array_equal(g["tr"][9:12], T0), i
             assert np.array_equal(g["tr"][12:21], R1.reshape(-1)) and np.array_equal(g["tr"][21:], T1), i
     assert hits > 100
+
+
+@pytest.mark.gpu
+def test_distance_sequence_matches_oracle(tmp_path):
+    """C2A_Distance through the C++ layer: 64 queries of the knot pair in
+    order, the models' last_tri carried between calls, every output equal to
+    the oracle's run of the same sequence."""
+    tris, _ = W.torus_knot()
+    t9 = np.ascontiguousarray(tris.reshape(-1, 9))
+    nq, rel, abs_ = 64, 0.05, 0.01
+    P = W.knot_queries(nq, 515, collide_frac=0.5)
+    poses = np.ascontiguousarray(np.concatenate([P[0], P[2]], axis=1))   # nq x 24
+    payload = (_i32(len(t9)) + t9.tobytes() + _i32(len(t9)) + t9.tobytes() +
+               np.array([rel, abs_]).tobytes() + _i32(nq) + poses.tobytes())
+    print("distance_demo: %d queries" % nq, flush=True)
+    b = _run("distance_demo", payload, tmp_path)
+    rec = np.dtype([("d7", "<f8", 7), ("i6", "<i4", 6), ("rt", "<f8", 12)])
+    got = np.frombuffer(b, rec)
+    assert len(got) == nq
+    m1, m2 = O.OModel(t9), O.OModel(t9)
+    ids = np.array(m1.tris()["id"])
+    for i in range(nq):
+        p = poses[i]
+        R1, T1, R2, T2 = p[:9].reshape(3, 3), p[9:12], p[12:21].reshape(3, 3), p[21:]
+        r = O.distance(m1, m2, R1, T1, R2, T2, rel, abs_)
+        g = got[i]
+        assert g["d7"][0] == r.distance, i
+        assert tuple(g["d7"][1:4]) == tuple(r.p1) and tuple(g["d7"][4:]) == tuple(r.p2), i
+        assert (g["i6"][0], g["i6"][1]) == (r.bv_tests, r.tri_tests), i
+        assert (g["i6"][4], g["i6"][5]) == (r.last_tri[0], r.last_tri[1]), i
+        assert (g["i6"][2], g["i6"][3]) == (ids[r.last_tri[0]], ids[r.last_tri[1]]), i
+        Rrel = R1.T @ R2
+        assert np.allclose(g["rt"][:9], Rrel.reshape(-1), atol=1e-12), i
+        assert np.allclose(g["rt"][9:], R1.T @ (T2 - T1), atol=1e-12), i
