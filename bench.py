@@ -48,6 +48,8 @@ def parse():
                     help="time the plain order, the learned order, or both (value = plain)")
     ap.add_argument("--scorer", default="", help="scorer file (default: the shipped one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-distance", action="store_true",
+                    help="skip the static distance-query figure (knot workload)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 256 MiB device copy (to scale HBM counters)")
     return ap.parse_args()
@@ -89,6 +91,45 @@ def cpu_baseline(W, nq_sample):
     return {"value": nq_sample / dt, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": "first %d queries of the same workload, %d threads, %.2f s wall"
                       % (nq_sample, threads, dt)}
+
+
+def distance_bench(meshes, tris, nq=65536, sample=4096, reps=3):
+    """Static distance queries (C2A_Distance, zero error bounds) between the
+    two knots at the start poses of the knot workload's motions: GPU
+    queries/s through the one-shot C ABI call (host arrays in and out, so the
+    figure includes the PCIe copies and the call's device allocations), and
+    the oracle on all host cores over the first `sample` queries."""
+    import numpy as np
+
+    import c2a_amd as G
+    from c2a_amd import workloads as Wm
+    from oracle import oracle as O
+    P = Wm.knot_queries(nq, 2024, collide_frac=0.3)
+    poses = np.ascontiguousarray(np.concatenate([P[0], P[2]], axis=1))
+    pairs = np.tile(np.array([0, 1], np.int32), (nq, 1))
+    ref = G.distance(meshes, poses, pairs)   # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        got = G.distance(meshes, poses, pairs)
+    gpu_s = (time.perf_counter() - t0) / reps
+    if not np.array_equal(ref["distance"], got["distance"]):
+        raise SystemExit("bench.py: distance results changed between calls")
+    threads = min(64, os.cpu_count() or 1)
+    m1, m2 = O.OModel(tris), O.OModel(tris)
+    t0 = time.perf_counter()
+    cpu = O.distance_batch(m1, m2, poses[:sample], nthreads=threads)
+    cpu_s = time.perf_counter() - t0
+    same = all(c.distance == g and c.bv_tests == b
+               for c, g, b in zip(cpu, got["distance"][:sample], got["bv_tests"][:sample]))
+    return {"value": nq / gpu_s, "unit": "queries/s", "nq": nq,
+            "timing": "host to host through c2a_distance (PCIe copies and allocations included)",
+            "bv_tests_mean": float(got["bv_tests"].mean()),
+            "zero_distance_fraction": float((got["distance"] == 0).mean()),
+            "matches_oracle_on_sample": bool(same),
+            "cpu_baseline": {"value": sample / cpu_s, "unit": "queries/s", "cores": threads,
+                             "kind": "port",
+                             "sample": "first %d queries, %d threads, %.2f s wall"
+                                       % (sample, threads, cpu_s)}}
 
 
 def overshoots(r):
@@ -281,6 +322,8 @@ def main():
         out["learned"] = learned
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, wl["sample"])
+    if args.workload == "knot" and not args.no_distance:
+        out["distance_query"] = distance_bench(meshes, wl["tris"][0])
     print(json.dumps(out), flush=True)
 
 

@@ -1866,3 +1866,57 @@ int c2o_batch_toc(const c2o_model *m1, const c2o_model *m2, int nq,
   pthread_mutex_destroy(&job.mu);
   return 0;
 }
+
+/* Independent distance queries over threads (the CPU baseline of the GPU
+ * batch): query i seeds the models' last_tri with seeds[2i], seeds[2i+1]. */
+typedef struct {
+  const c2o_model *m1, *m2;
+  const double *poses;
+  const int *seeds;
+  double rel_err, abs_err;
+  c2o_distance_result *out;
+  int nq, next;
+  pthread_mutex_t mu;
+} distance_job;
+
+static void distance_one(distance_job *job, int i, c2o_model *w1, c2o_model *w2) {
+  m3 R1, R2;
+  double T1[3], T2[3];
+  pose_unpack(job->poses + 24 * (size_t)i, R1, T1);
+  pose_unpack(job->poses + 24 * (size_t)i + 12, R2, T2);
+  w1->last_tri = job->seeds ? job->seeds[2 * i] : 0;
+  w2->last_tri = job->seeds ? job->seeds[2 * i + 1] : 0;
+  c2o_distance(w1, w2, R1, T1, R2, T2, job->rel_err, job->abs_err, &job->out[i]);
+}
+
+static void *distance_worker(void *arg) {
+  distance_job *job = (distance_job *)arg;
+  c2o_model w1 = *job->m1, w2 = *job->m2;   /* only last_tri is written */
+  for (;;) {
+    pthread_mutex_lock(&job->mu);
+    int i = job->next++;
+    pthread_mutex_unlock(&job->mu);
+    if (i >= job->nq) break;
+    distance_one(job, i, &w1, &w2);
+  }
+  return NULL;
+}
+
+void c2o_distance_batch(const c2o_model *m1, const c2o_model *m2, int nq, const double *poses24,
+                        const int *seeds, double rel_err, double abs_err, int nthreads,
+                        c2o_distance_result *out) {
+  distance_job job;
+  job.m1 = m1; job.m2 = m2; job.poses = poses24; job.seeds = seeds;
+  job.rel_err = rel_err; job.abs_err = abs_err; job.out = out; job.nq = nq; job.next = 0;
+  if (nthreads <= 1) {
+    c2o_model w1 = *m1, w2 = *m2;
+    for (int i = 0; i < nq; i++) distance_one(&job, i, &w1, &w2);
+    return;
+  }
+  pthread_mutex_init(&job.mu, NULL);
+  pthread_t th[256];
+  if (nthreads > 256) nthreads = 256;
+  for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, distance_worker, &job);
+  for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  pthread_mutex_destroy(&job.mu);
+}

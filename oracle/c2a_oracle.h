@@ -178,6 +178,11 @@ typedef struct {
 void c2o_distance(c2o_model *m1, c2o_model *m2, const double R1[3][3], const double T1[3],
                   const double R2[3][3], const double T2[3], double rel_err, double abs_err,
                   c2o_distance_result *r);
+/* nq independent queries (poses24: model 1's pose then model 2's, row-major
+ * R then T), query i seeded with seeds[2i], seeds[2i+1] (NULL: triangle 0). */
+void c2o_distance_batch(const c2o_model *m1, const c2o_model *m2, int nq, const double *poses24,
+                        const int *seeds, double rel_err, double abs_err, int nthreads,
+                        c2o_distance_result *out);
 
 /* C2A_Solve (C2A.cpp:2315-2444): tol_d = tol_t = 1e-4, contact threshold
  * 2d+0.001.  Seeds from m*->last_tri like the demo (mainTorusknot.cpp:314). */

@@ -102,6 +102,10 @@ def lib():
         L.c2o_distance.restype = None
         L.c2o_distance.argtypes = [C.POINTER(Model), C.POINTER(Model), dp, dp, dp, dp,
                                    C.c_double, C.c_double, C.POINTER(DistanceResult)]
+        L.c2o_distance_batch.restype = None
+        L.c2o_distance_batch.argtypes = [C.POINTER(Model), C.POINTER(Model), C.c_int, dp, ip,
+                                         C.c_double, C.c_double, C.c_int,
+                                         C.POINTER(DistanceResult)]
         L.c2o_toc_step.restype = None
         L.c2o_toc_step.argtypes = [C.POINTER(Model), C.POINTER(Model), C.POINTER(Motion),
                                    C.POINTER(Motion), dp, dp, dp, dp, C.POINTER(Model),
@@ -306,6 +310,18 @@ def distance(m1, m2, R1, T1, R2, T2, rel_err=0.0, abs_err=0.0, seeds=None):
     lib().c2o_distance(C.byref(m1.m), C.byref(m2.m), _dp(f64(R1)), _dp(f64(T1)), _dp(f64(R2)),
                        _dp(f64(T2)), float(rel_err), float(abs_err), C.byref(r))
     return r
+
+
+def distance_batch(m1, m2, poses24, seeds=None, rel_err=0.0, abs_err=0.0, nthreads=1):
+    """Independent distance queries; returns the list of DistanceResult."""
+    P = f64(poses24).reshape(-1, 24)
+    nq = len(P)
+    out = (DistanceResult * nq)()
+    sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.int32).reshape(nq, 2)
+    lib().c2o_distance_batch(C.byref(m1.m), C.byref(m2.m), nq, _dp(P),
+                             None if sd is None else _ip(sd), float(rel_err), float(abs_err),
+                             int(nthreads), out)
+    return list(out)
 
 
 def toc_step(m1, m2, mo1, mo2, R1, T1, R2, T2, seeds, numCA, mint_prev, upbound):
