@@ -373,17 +373,10 @@ def demo_queries(path=None):
     frames below 101, (101, 102) below 202, (202, 203) after); object 2
     keeps the rotation of frame `iframe` of animation 2 and moves between
     the translations of frames step1 and step2.  Returns (P00, P01, P10, P11)."""
+    from c2a_amd import loaders
     d = np.load(path or os.path.join(_FIXTURES, "torusknot_ani.npz"))
     R, T = d["R"], d["T"]
-    n = R.shape[1]
-    P = [np.zeros((n, 12)) for _ in range(4)]
-    for f in range(n):
-        s1, s2 = (0, 1) if f < 101 else ((101, 102) if f < 202 else (202, 203))
-        P[0][f] = np.concatenate([R[0][s1].reshape(9), T[0][s1]])
-        P[1][f] = np.concatenate([R[0][s2].reshape(9), T[0][s2]])
-        P[2][f] = np.concatenate([R[1][f].reshape(9), T[1][s1]])
-        P[3][f] = np.concatenate([R[1][f].reshape(9), T[1][s2]])
-    return P
+    return loaders.demo_queries(R[0], T[0], R[1], T[1])
 
 
 CONFIG_DEMO = dict(name="reference demo: bunny vs bunny, torus-knot animation", nq=303,

@@ -20,30 +20,9 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
-
-def read_tri(path):
-    tok = open(path).read().split()
-    assert tok[0] == "TRI"
-    nv, nt = int(tok[1]), int(tok[2])
-    v = np.array(tok[3:3 + 3 * nv], dtype=np.float64).reshape(nv, 3)
-    f = np.array(tok[3 + 3 * nv:3 + 3 * nv + 3 * nt], dtype=np.int32).reshape(nt, 3)
-    return v, f
-
-
-def read_ani(path):
-    tok = open(path).read().split()
-    nf = int(float(tok[0].rstrip("f")))
-    R = np.zeros((nf, 3, 3))
-    T = np.zeros((nf, 3))
-    k = 1
-    while k + 12 < len(tok):
-        i = int(float(tok[k].rstrip("f")))
-        vals = np.array(tok[k + 1:k + 13], dtype=np.float64)
-        R[i] = vals[:9].reshape(3, 3).T   # stored column by column
-        T[i] = vals[9:]
-        k += 13
-    return R, T
+from c2a_amd.loaders import read_ani, read_tri  # noqa: E402
 
 
 def main():
