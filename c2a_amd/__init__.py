@@ -82,7 +82,8 @@ _lib = None
 EXPORTS = ["c2a_params_default", "c2a_mesh_create", "c2a_mesh_destroy", "c2a_mesh_info",
            "c2a_mesh_tri_ids", "c2a_mesh_export", "c2a_ccd_batch", "c2a_batch_create",
            "c2a_batch_run", "c2a_batch_fetch", "c2a_batch_destroy", "c2a_batch_last_ms",
-           "c2a_ccd_step", "c2a_contact_count", "c2a_distance", "c2a_motion_prepare", "c2a_motion_pose",
+           "c2a_ccd_step", "c2a_contact_count", "c2a_distance", "c2a_collide_distance",
+           "c2a_motion_prepare", "c2a_motion_pose",
            "c2a_status_string", "c2a_last_error", "c2a_scorer_create", "c2a_scorer_destroy",
            "c2a_scorer_set_params", "c2a_scorer_get_params", "c2a_scorer_infer",
            "c2a_scorer_grad", "c2a_scorer_train", "c2a_scorer_save", "c2a_scorer_load",
@@ -123,6 +124,7 @@ def lib():
     L.c2a_contact_count.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, dp, i64, ip, vp]
     L.c2a_distance.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, ip, C.c_double, C.c_double,
                                i64, vp, vp]
+    L.c2a_collide_distance.argtypes = L.c2a_distance.argtypes
     L.c2a_motion_prepare.argtypes = [dp, dp, dp]
     L.c2a_motion_pose.argtypes = [dp, C.c_double, dp, dp]
     L.c2a_status_string.argtypes = [C.c_int]
@@ -293,9 +295,12 @@ def contact_count(meshes, poses, mesh_pairs, thresholds):
     return out
 
 
-def distance(meshes, poses, mesh_pairs, seed_tris=None, rel_err=0.0, abs_err=0.0):
+def distance(meshes, poses, mesh_pairs, seed_tris=None, rel_err=0.0, abs_err=0.0,
+             overlap_only=False):
     """Static distance queries (C2A_Distance): poses (nq, 24) = pose of mesh 1
-    then mesh 2 (R row-major, T); returns a DISTANCE_DTYPE array."""
+    then mesh 2 (R row-major, T); returns a DISTANCE_DTYPE array.
+    overlap_only=True: C2A_Collide's distance form (only pairs whose RSS
+    volumes overlap are visited)."""
     poses = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 24)
     nq = len(poses)
     mp = np.ascontiguousarray(mesh_pairs, dtype=np.int32).reshape(nq, 2)
@@ -303,9 +308,10 @@ def distance(meshes, poses, mesh_pairs, seed_tris=None, rel_err=0.0, abs_err=0.0
     if seed_tris is not None:
         seeds = np.ascontiguousarray(seed_tris, dtype=np.int32).reshape(nq, 2)
     out = np.zeros(nq, dtype=DISTANCE_DTYPE)
-    _check(lib().c2a_distance(_mesh_table(meshes), len(meshes), _dp(poses), _ip(mp),
-                              _ip(seeds) if seeds is not None else None, rel_err, abs_err, nq,
-                              out.ctypes.data, None))
+    f = lib().c2a_collide_distance if overlap_only else lib().c2a_distance
+    _check(f(_mesh_table(meshes), len(meshes), _dp(poses), _ip(mp),
+             _ip(seeds) if seeds is not None else None, rel_err, abs_err, nq, out.ctypes.data,
+             None))
     return out
 
 

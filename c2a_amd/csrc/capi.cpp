@@ -621,9 +621,10 @@ int c2a_contact_count(c2a_mesh* const* meshes, int32_t nmesh, const double* pose
   return C2A_OK;
 }
 
-int c2a_distance(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
-                 const int32_t* mesh_pairs, const int32_t* seed_tris, double rel_err,
-                 double abs_err, int64_t nq, c2a_distance_result* out, void* stream) {
+static int distance_queries(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                            const int32_t* mesh_pairs, const int32_t* seed_tris, double rel_err,
+                            double abs_err, int64_t nq, c2a_distance_result* out, void* stream,
+                            int overlap_only) {
   if (nq < 0 || (nq > 0 && (!poses || !mesh_pairs || !out)))
     return fail(C2A_ERR_BAD_ARG, "bad arrays");
   if (!(rel_err >= 0) || !(abs_err >= 0))
@@ -650,6 +651,7 @@ int c2a_distance(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
     unpack_pose(poses + 24 * i + 12, c.R2, c.T2);
     c.rel_err = rel_err;
     c.abs_err = abs_err;
+    c.overlap_only = overlap_only;
   }
   int CAP = pick_cap(tab, mesh_pairs, nq, 2);
   if (CAP < 0) return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
@@ -676,6 +678,20 @@ int c2a_distance(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
                          hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return C2A_OK;
+}
+
+int c2a_distance(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                 const int32_t* mesh_pairs, const int32_t* seed_tris, double rel_err,
+                 double abs_err, int64_t nq, c2a_distance_result* out, void* stream) {
+  return distance_queries(meshes, nmesh, poses, mesh_pairs, seed_tris, rel_err, abs_err, nq, out,
+                          stream, 0);
+}
+
+int c2a_collide_distance(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                         const int32_t* mesh_pairs, const int32_t* seed_tris, double rel_err,
+                         double abs_err, int64_t nq, c2a_distance_result* out, void* stream) {
+  return distance_queries(meshes, nmesh, poses, mesh_pairs, seed_tris, rel_err, abs_err, nq, out,
+                          stream, 1);
 }
 
 int c2a_motion_prepare(const double* pose_start, const double* pose_end, double* motion24) {

@@ -279,6 +279,7 @@ struct Leader {
   int spos, nev, logfull;        // scan position, events, log capacity reached
   int napp;                      // next free record pair of the arena
   int need_score;                // replayed records wait for the scorer
+  int overlap_only;              // distance query restricted to overlapping RSS pairs
   unsigned char* ab;             // this wavefront's replay arena
 };
 
@@ -1345,7 +1346,11 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
       bool desc;
       if (contact)
         desc = (L.mode == MODE_DISTANCE)   // C2A_PQP.cpp:594-617
-                   ? ((c.d < (L.D - L.abs_err)) || (c.d * (1 + L.rel_err) < L.D))
+                   ? ((c.d < (L.D - L.abs_err)) || (c.d * (1 + L.rel_err) < L.D)) &&
+                         // C2A_Collide's overlap test (C2A_PQP.cpp:1068-1070):
+                         // RectDist <= r1 + r2 exactly when the clamped RSS
+                         // distance is 0
+                         (!L.overlap_only || c.d == 0.0)
                    : c.d < L.thr;   // abs = rel = 0 (C2A.cpp:1760-1761)
       else desc = c.mt < L.Up && ((c.d < (L.D - L.abs_err)) || (c.d * (1 + L.rel_err) < L.D));
       if (!desc) {
@@ -1539,6 +1544,14 @@ C2A_HD void begin_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   L.D = tri_distance(Rrel, Trel, ta, tb, p, q, nullptr);
   if (L.mode == MODE_DISTANCE) {   // C2A_Distance's first bound (C2A_PQP.cpp:998-1005)
     for (int i = 0; i < 3; i++) { L.p1[i] = p[i]; L.p2[i] = q[i]; }
+    if (L.overlap_only) {   // C2A_Collide tests the root pair too (C2A_PQP.cpp:1068)
+      double Rr[9], Tr[3], d, mt;
+      for (int i = 0; i < 9; i++) Rr[i] = g.stack[0].R[i];
+      for (int i = 0; i < 3; i++) Tr[i] = g.stack[0].T[i];
+      (void)pair_values(nodes_of(M1), nodes_of(M2), 0, 0, Rr, Tr, g.r1, g.mo[0], g.mo[1], true,
+                        d, mt);
+      if (d != 0.0) L.sp = 0;
+    }
     L.state = ST_CONTACT_WALK;
     return;
   }
@@ -1752,6 +1765,7 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     L.num_contact = 0;
     L.rel_err = c.rel_err;
     L.abs_err = c.abs_err;
+    L.overlap_only = c.overlap_only;
     L.seed1 = c.seed[0];
     L.seed2 = c.seed[1];
     L.last1 = c.seed[0];

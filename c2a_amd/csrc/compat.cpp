@@ -340,9 +340,11 @@ int C2A_TimeOfContactStep(CInterpMotion* m1, CInterpMotion* m2, C2A_TimeOfContac
   return PQP_OK;
 }
 
-int C2A_Distance(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
-                 PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,
-                 PQP_REAL abs_err, int qsize) {
+namespace {
+
+int distance_query(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
+                   PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,
+                   PQP_REAL abs_err, int qsize, bool overlap_only) {
   if (o1->build_state != 2 || o2->build_state != 2) return PQP_ERR_UNPROCESSED_MODEL;
   const auto t0 = std::chrono::steady_clock::now();
   c2a_mesh* meshes[2] = {o1->engine_mesh(), o2->engine_mesh()};
@@ -352,8 +354,12 @@ int C2A_Distance(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A
   const int32_t pairs[2] = {0, 1};
   const int32_t seeds[2] = {o1->tri_index(o1->last_tri), o2->tri_index(o2->last_tri)};
   c2a_distance_result r;
-  if (c2a_distance(meshes, 2, poses, pairs, seeds, rel_err, abs_err, 1, &r, nullptr) != C2A_OK)
-    return -1;
+  const int rc = overlap_only
+                     ? c2a_collide_distance(meshes, 2, poses, pairs, seeds, rel_err, abs_err, 1,
+                                            &r, nullptr)
+                     : c2a_distance(meshes, 2, poses, pairs, seeds, rel_err, abs_err, 1, &r,
+                                    nullptr);
+  if (rc != C2A_OK) return -1;
   // [R, T] = [R1, T1]^T [R2, T2] (C2A_PQP.cpp:990-993)
   for (int i = 0; i < 3; i++) {
     for (int j = 0; j < 3; j++)
@@ -378,6 +384,20 @@ int C2A_Distance(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A
   res->query_time_secs =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return PQP_OK;
+}
+
+}  // namespace
+
+int C2A_Distance(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
+                 PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,
+                 PQP_REAL abs_err, int qsize) {
+  return distance_query(res, R1, T1, o1, R2, T2, o2, rel_err, abs_err, qsize, false);
+}
+
+int C2A_Collide(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
+                PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,
+                PQP_REAL abs_err, int qsize) {
+  return distance_query(res, R1, T1, o1, R2, T2, o2, rel_err, abs_err, qsize, true);
 }
 
 PQP_REAL C2A_QueryContact(CInterpMotion* m1, CInterpMotion* m2, C2A_TimeOfContactResult* res,

@@ -118,6 +118,13 @@ int C2A_Distance(C2A_DistanceResult* result, PQP_REAL R1[3][3], PQP_REAL T1[3], 
                  PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,
                  PQP_REAL abs_err, int qsize = 2);
 
+// C2A_Distance restricted to pairs whose RSS volumes overlap, the root pair
+// included (C2A.h:263-267, C2A_PQP.cpp:1199-1279).  Returns PQP_OK,
+// PQP_ERR_UNPROCESSED_MODEL, or -1 when the engine reports an error.
+int C2A_Collide(C2A_DistanceResult* result, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
+                PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,
+                PQP_REAL abs_err, int qsize = 2);
+
 // Counts the triangle pairs within `threshold` at the motions' current poses.
 PQP_REAL C2A_QueryContact(CInterpMotion* objmotion1, CInterpMotion* objmotion2,
                           C2A_TimeOfContactResult* res, C2A_Model* o1, C2A_Model* o2,

@@ -221,6 +221,15 @@ int c2a_distance(c2a_mesh *const *meshes, int32_t nmesh, const double *poses /* 
                  double rel_err, double abs_err, int64_t nq, c2a_distance_result *out,
                  void *stream);
 
+/* The same search entering only pairs whose RSS volumes overlap, the root
+ * pair included: C2A_Collide's C2A_DistanceResult form
+ * (C2A/src/C2A_PQP.cpp:1199-1279 -> C2ACollideRecurse :1059-1193).  Pairs
+ * farther apart are never visited, so the result is the seed pair's distance
+ * unless an overlapping pair is closer. */
+int c2a_collide_distance(c2a_mesh *const *meshes, int32_t nmesh, const double *poses,
+                         const int32_t *mesh_pairs, const int32_t *seed_tris, double rel_err,
+                         double abs_err, int64_t nq, c2a_distance_result *out, void *stream);
+
 /* Linear motion between two poses, as the kernel integrates it
  * (CInterpMotion_Linear ctor + velocity(), InterpMotion.cpp:148-270,
  * 486-513): motion24 = R0[9], T0[3], q0[4] (x,y,z,w), cv[3], axis[3],

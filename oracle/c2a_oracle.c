@@ -1661,6 +1661,7 @@ typedef struct {
   m3 Rrel;
   double Trel[3];
   double rel_err, abs_err;
+  int overlap_only;
   c2o_distance_result *r;
 } distance_ctx;
 
@@ -1671,6 +1672,12 @@ static void distance_recurse(distance_ctx *cx, const m3 R, const double T[3], in
   const c2o_bv *B1 = &cx->m1->bvs[b1], *B2 = &cx->m2->bvs[b2];
   int l1 = B1->first_child < 0, l2 = B2->first_child < 0;
   c2o_distance_result *r = cx->r;
+  /* C2ACollideRecurse enters a pair only if C2A_BV_Overlap holds
+   * (C2A_PQP.cpp:1068-1070 -> C2A_BV.cpp:641-650: RectDist <= r1 + r2) */
+  if (cx->overlap_only) {
+    double P[3], Q[3], S[3];
+    if (!(c2o_rect_dist(R, T, B1->l, B2->l, P, Q, S) <= B1->r + B2->r)) return;
+  }
   if (l1 && l2) {
     double p[3], q[3];
     int t1 = -B1->first_child - 1, t2 = -B2->first_child - 1;
@@ -1715,10 +1722,12 @@ static void distance_recurse(distance_ctx *cx, const m3 R, const double T[3], in
 #undef ENTER
 }
 
-void c2o_distance(c2o_model *m1, c2o_model *m2, const double R1[3][3], const double T1[3],
-                  const double R2[3][3], const double T2[3], double rel_err, double abs_err,
-                  c2o_distance_result *r) {
+static void distance_query(c2o_model *m1, c2o_model *m2, const double R1[3][3],
+                           const double T1[3], const double R2[3][3], const double T2[3],
+                           double rel_err, double abs_err, int overlap_only,
+                           c2o_distance_result *r) {
   distance_ctx cx;
+  cx.overlap_only = overlap_only;
   m3 R;
   double T[3], p[3], q[3];
   int ta, fa, tb, fb;
@@ -1738,6 +1747,20 @@ void c2o_distance(c2o_model *m1, c2o_model *m2, const double R1[3][3], const dou
   mtxv(r->p2, cx.Rrel, u);
   r->last_tri[0] = m1->last_tri;
   r->last_tri[1] = m2->last_tri;
+}
+
+void c2o_distance(c2o_model *m1, c2o_model *m2, const double R1[3][3], const double T1[3],
+                  const double R2[3][3], const double T2[3], double rel_err, double abs_err,
+                  c2o_distance_result *r) {
+  distance_query(m1, m2, R1, T1, R2, T2, rel_err, abs_err, 0, r);
+}
+
+/* C2A_Collide, C2A_DistanceResult form (C2A_PQP.cpp:1199-1279): the distance
+ * search entering only overlapping RSS pairs, the root pair included. */
+void c2o_collide_distance(c2o_model *m1, c2o_model *m2, const double R1[3][3],
+                          const double T1[3], const double R2[3][3], const double T2[3],
+                          double rel_err, double abs_err, c2o_distance_result *r) {
+  distance_query(m1, m2, R1, T1, R2, T2, rel_err, abs_err, 1, r);
 }
 
 /* ------------------------------------------------------------------------ */

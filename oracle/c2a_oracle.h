@@ -178,6 +178,11 @@ typedef struct {
 void c2o_distance(c2o_model *m1, c2o_model *m2, const double R1[3][3], const double T1[3],
                   const double R2[3][3], const double T2[3], double rel_err, double abs_err,
                   c2o_distance_result *r);
+/* C2A_Collide, C2A_DistanceResult form (C2A_PQP.cpp:1199-1279): the same
+ * search entering only pairs whose RSS volumes overlap, the root included. */
+void c2o_collide_distance(c2o_model *m1, c2o_model *m2, const double R1[3][3],
+                          const double T1[3], const double R2[3][3], const double T2[3],
+                          double rel_err, double abs_err, c2o_distance_result *r);
 /* nq independent queries (poses24: model 1's pose then model 2's, row-major
  * R then T), query i seeded with seeds[2i], seeds[2i+1] (NULL: triangle 0). */
 void c2o_distance_batch(const c2o_model *m1, const c2o_model *m2, int nq, const double *poses24,

@@ -102,6 +102,8 @@ def lib():
         L.c2o_distance.restype = None
         L.c2o_distance.argtypes = [C.POINTER(Model), C.POINTER(Model), dp, dp, dp, dp,
                                    C.c_double, C.c_double, C.POINTER(DistanceResult)]
+        L.c2o_collide_distance.restype = None
+        L.c2o_collide_distance.argtypes = L.c2o_distance.argtypes
         L.c2o_distance_batch.restype = None
         L.c2o_distance_batch.argtypes = [C.POINTER(Model), C.POINTER(Model), C.c_int, dp, ip,
                                          C.c_double, C.c_double, C.c_int,
@@ -301,13 +303,16 @@ class DistanceResult(C.Structure):
                 ("last_tri", C.c_int * 2)]
 
 
-def distance(m1, m2, R1, T1, R2, T2, rel_err=0.0, abs_err=0.0, seeds=None):
-    """C2A_Distance (qsize <= 2) at explicit poses.  seeds sets the models'
-    last_tri first (build order); both are updated like the reference's."""
+def distance(m1, m2, R1, T1, R2, T2, rel_err=0.0, abs_err=0.0, seeds=None,
+             overlap_only=False):
+    """C2A_Distance (qsize <= 2) at explicit poses, or with overlap_only
+    C2A_Collide's distance form.  seeds sets the models' last_tri first
+    (build order); both are updated like the reference's."""
     if seeds is not None:
         m1.last_tri, m2.last_tri = int(seeds[0]), int(seeds[1])
     r = DistanceResult()
-    lib().c2o_distance(C.byref(m1.m), C.byref(m2.m), _dp(f64(R1)), _dp(f64(T1)), _dp(f64(R2)),
+    f = lib().c2o_collide_distance if overlap_only else lib().c2o_distance
+    f(C.byref(m1.m), C.byref(m2.m), _dp(f64(R1)), _dp(f64(T1)), _dp(f64(R2)),
                        _dp(f64(T2)), float(rel_err), float(abs_err), C.byref(r))
     return r
 

@@ -175,7 +175,8 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
 // Static distance queries (MODE_DISTANCE) between two meshes, one lane.
 extern "C" int emu_distance(const double* tris1, int64_t n1, const double* tris2, int64_t n2,
                             const double* poses, const int32_t* seeds, double rel_err,
-                            double abs_err, int64_t nq, c2a_distance_result* out) {
+                            double abs_err, int64_t nq, c2a_distance_result* out,
+                            int overlap_only) {
   Bvh b1, b2;
   if (build_bvh(tris1, nullptr, nullptr, n1, &b1) || build_bvh(tris2, nullptr, nullptr, n2, &b2))
     return 1;
@@ -196,6 +197,7 @@ extern "C" int emu_distance(const double* tris1, int64_t n1, const double* tris2
     memcpy(c.R2, poses + 24 * i + 12, 72); memcpy(c.T2, poses + 24 * i + 21, 24);
     c.rel_err = rel_err;
     c.abs_err = abs_err;
+    c.overlap_only = overlap_only;
   }
   int64_t counter = 0;
   KArgs a;

@@ -51,11 +51,11 @@ def world(R, T, p):
     return np.asarray(R) @ np.asarray(p) + np.asarray(T)
 
 
-def oracle_batch(t1, t2, cases, rel, abs_, seeds):
+def oracle_batch(t1, t2, cases, rel, abs_, seeds, overlap_only=False):
     m1, m2 = O.OModel(t1), O.OModel(t2)
     out = []
     for (R1, T1, R2, T2), s in zip(cases, seeds):
-        r = O.distance(m1, m2, R1, T1, R2, T2, rel, abs_, seeds=s)
+        r = O.distance(m1, m2, R1, T1, R2, T2, rel, abs_, seeds=s, overlap_only=overlap_only)
         out.append((r.distance, tuple(r.p1), tuple(r.p2), r.bv_tests, r.tri_tests,
                     r.last_tri[0], r.last_tri[1]))
     return out
@@ -144,6 +144,32 @@ def test_oracle_overlap_is_zero(spheres):
     assert r.distance == 0.0
 
 
+def test_oracle_collide_form(spheres):
+    """C2A_Collide's distance form enters only overlapping RSS pairs: with
+    the two spheres far apart nothing is visited and the seed pair's distance
+    stands; with them interpenetrating the zero distance is found (every
+    ancestor of an intersecting triangle pair overlaps); in between, the
+    result is never below the true minimum."""
+    t1, t2 = spheres
+    rng = np.random.default_rng(23)
+    m1, m2 = O.OModel(t1), O.OModel(t2)
+    tris1, tris2 = m1.tri_xyz(), m2.tri_xyz()   # build order
+    for sep in (6.0, 1.0, 0.3, 2.05):
+        R1, T1, R2, T2 = random_case(rng, sep)
+        seeds = (int(rng.integers(len(t1))), int(rng.integers(len(t2))))
+        r = O.distance(m1, m2, R1, T1, R2, T2, 0.0, 0.0, seeds=seeds, overlap_only=True)
+        exact = B.mesh_min_distance(t1, t2, R1, T1, R2, T2)[0]
+        assert r.distance >= exact - 1e-10 * max(1.0, exact), sep
+        if sep == 6.0:
+            A = tris1[seeds[0]].reshape(3, 3) @ R1.T + T1
+            Bt = tris2[seeds[1]].reshape(3, 3) @ R2.T + T2
+            assert abs(r.distance - B.poly_poly(A, Bt)) <= 1e-10
+            assert (r.bv_tests, r.tri_tests) == (0, 0)
+            assert (r.last_tri[0], r.last_tri[1]) == seeds
+        if exact == 0.0:
+            assert r.distance == 0.0, sep
+
+
 # ------------------------------------------------- kernel logic on the CPU
 
 @pytest.fixture(scope="module")
@@ -154,14 +180,14 @@ def emu():
     return C.CDLL(EMU)
 
 
-def run_emu(emu, t1, t2, cases, rel, abs_, seeds):
+def run_emu(emu, t1, t2, cases, rel, abs_, seeds, overlap_only=0):
     P = poses24(cases)
     s = np.ascontiguousarray(seeds, dtype=np.int32).reshape(-1, 2)
     out = np.zeros(len(cases), dtype=G.DISTANCE_DTYPE)
     a, b = np.ascontiguousarray(t1), np.ascontiguousarray(t2)
     rc = emu.emu_distance(G._dp(a), C.c_int64(len(a)), G._dp(b), C.c_int64(len(b)), G._dp(P),
                           G._ip(s), C.c_double(rel), C.c_double(abs_), C.c_int64(len(cases)),
-                          out.ctypes.data_as(C.c_void_p))
+                          out.ctypes.data_as(C.c_void_p), int(overlap_only))
     assert rc == 0
     return out
 
@@ -174,6 +200,17 @@ def test_kernel_logic_distance_matches_oracle(emu, knot_tris, rel, abs_):
     seeds = rng.integers(0, len(t9), (12, 2))
     ref = oracle_batch(t9, t9, cases, rel, abs_, seeds)
     check_same(ref, run_emu(emu, t9, t9, cases, rel, abs_, seeds))
+
+
+@pytest.mark.parametrize("rel,abs_", [(0.0, 0.0), (0.5, 0.02)])
+def test_kernel_logic_collide_form_matches_oracle(emu, knot_tris, rel, abs_):
+    t9, _ = knot_tris
+    rng = np.random.default_rng(61 + int(rel * 10))
+    cases = [random_case(rng, s) for s in rng.uniform(0.0, 2.5, 12)]
+    seeds = rng.integers(0, len(t9), (12, 2))
+    ref = oracle_batch(t9, t9, cases, rel, abs_, seeds, overlap_only=True)
+    check_same(ref, run_emu(emu, t9, t9, cases, rel, abs_, seeds, overlap_only=1))
+    assert any(e[3] == 0 for e in ref) and any(e[3] > 0 for e in ref)
 
 
 def test_kernel_logic_distance_unequal_meshes(emu, spheres, knot_tris):
@@ -202,6 +239,22 @@ def test_gpu_distance_bit_exact(knot_tris, rel, abs_):
     got = G.distance(meshes, poses24(cases), np.tile([0, 1], (nq, 1)), seeds, rel, abs_)
     check_same(ref, got)
     assert (got["distance"] == 0).any() and (got["distance"] > 0).any()
+
+
+@pytest.mark.gpu
+def test_gpu_collide_form_bit_exact(knot_tris):
+    t9, idx = knot_tris
+    nq = 128
+    rng = np.random.default_rng(71)
+    cases = [random_case(rng, s) for s in rng.uniform(0.0, 2.5, nq)]
+    seeds = rng.integers(0, len(t9), (nq, 2))
+    ref = oracle_batch(t9, t9, cases, 0.0, 0.0, seeds, overlap_only=True)
+    print("collide form: %d queries" % nq, flush=True)
+    meshes = [G.Mesh(t9, vidx=idx), G.Mesh(t9, vidx=idx)]
+    got = G.distance(meshes, poses24(cases), np.tile([0, 1], (nq, 1)), seeds,
+                     overlap_only=True)
+    check_same(ref, got)
+    assert (got["bv_tests"] == 0).any() and (got["bv_tests"] > 0).any()
 
 
 @pytest.mark.gpu
