@@ -16,6 +16,7 @@
 #include "../../include/c2a_amd.h"
 #include "bvh.h"
 #include "c2a_math.h"
+#include "ccd_core.h"
 #include "ccd_kernel.h"
 #include "scorer.h"
 
@@ -652,6 +653,12 @@ static int distance_queries(c2a_mesh* const* meshes, int32_t nmesh, const double
     c.rel_err = rel_err;
     c.abs_err = abs_err;
     c.overlap_only = overlap_only;
+    // C2A_Collide's root overlap test (C2A_PQP.cpp:1068), once per query
+    if (overlap_only &&
+        root_rss_distance(meshes[c.mesh[0]]->host.nodes.data(),
+                          meshes[c.mesh[1]]->host.nodes.data(), c.R1, c.T1, c.R2,
+                          c.T2) != 0.0)
+      c.overlap_only = 2;
   }
   int CAP = pick_cap(tab, mesh_pairs, nq, 2);
   if (CAP < 0) return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");

@@ -22,6 +22,7 @@
 #pragma once
 
 #include <stdint.h>
+#include <string.h>
 
 #include "bvh.h"
 #include "c2a_math.h"
@@ -213,7 +214,9 @@ struct Wave {
   Motion mo[2];
   int32_t mesh[2];
   MeshDev md[2];             // the two meshes' table entries (set at fetch)
-  int32_t contact;           // 1: contact-enumeration traversal (no motion bounds)
+  int32_t contact;           // 1: traversal without motion bounds (contact
+                             // enumeration, distance query); 2: a distance
+                             // query entering only overlapping RSS pairs
   int32_t gen;
   HotRec win[WIN];           // replayed records [L.wbase, L.wbase + WIN)
   LogEnt lbuf[LBUF];         // log entries [L.nflush, L.nlog) not yet in the arena
@@ -279,7 +282,6 @@ struct Leader {
   int spos, nev, logfull;        // scan position, events, log capacity reached
   int napp;                      // next free record pair of the arena
   int need_score;                // replayed records wait for the scorer
-  int overlap_only;              // distance query restricted to overlapping RSS pairs
   unsigned char* ab;             // this wavefront's replay arena
 };
 
@@ -716,6 +718,29 @@ C2A_HD void start_traversal(Wave<K, CAP>& g, Leader& L, const MeshDev& M1, const
   }
   lane_sync();
   L.sp = 1;
+}
+
+// RSS distance of the two root volumes at absolute poses: start_traversal's
+// root transform and pair_values' distance, evaluated by the host when it
+// sets up a C2A_Collide query (its root overlap test, C2A_PQP.cpp:1068).
+inline double root_rss_distance(const GpuNode* r1n, const GpuNode* r2n, const double* R1,
+                                const double* T1, const double* R2, const double* T2) {
+  double Tt[3], Rt[9], R[9], T[3], Rrel[9], Trel[3], C1R[9], C2R[9], C1T[3], C2T[3];
+  for (int i = 0; i < 9; i++) { C1R[i] = r1n->R[i]; C2R[i] = r2n->R[i]; }
+  for (int i = 0; i < 3; i++) { C1T[i] = r1n->Tr[i]; C2T[i] = r2n->Tr[i]; }
+  mtxm(Rrel, R1, R2);
+  sub3(Tt, T2, T1);
+  mtxv(Trel, R1, Tt);
+  mxm(Rt, Rrel, C2R);
+  mtxm(R, C1R, Rt);
+  mxvpv(Tt, Rrel, C2T, Trel);
+  sub3(Tt, Tt, C1T);
+  mtxv(T, C1R, Tt);
+  Motion none;
+  memset(&none, 0, sizeof(none));
+  double d, mt;
+  (void)pair_values((node_ptr)r1n, (node_ptr)r2n, 0, 0, R, T, R1, none, none, true, d, mt);
+  return d;
 }
 
 // Wavefront primitives of the scan (the host harness runs one lane).
@@ -1325,7 +1350,8 @@ C2A_HD void push_back(Wave<K, CAP>& g, Leader& L, const Cur& c, int where, int c
 // across lanes.  Returns true when a round has been requested.
 template <int K, int CAP>
 C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
-  const bool contact = g.contact != 0;
+  const int cmode = g.contact;
+  const bool contact = cmode != 0;
   const bool learned = a.scorer.on && !contact;
   const int gen = uni(g.gen);
   const ArenaView av = arena_view(L.ab, a.logcap);
@@ -1350,7 +1376,7 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
                          // C2A_Collide's overlap test (C2A_PQP.cpp:1068-1070):
                          // RectDist <= r1 + r2 exactly when the clamped RSS
                          // distance is 0
-                         (!L.overlap_only || c.d == 0.0)
+                         (cmode != 2 || c.d == 0.0)
                    : c.d < L.thr;   // abs = rel = 0 (C2A.cpp:1760-1761)
       else desc = c.mt < L.Up && ((c.d < (L.D - L.abs_err)) || (c.d * (1 + L.rel_err) < L.D));
       if (!desc) {
@@ -1544,14 +1570,6 @@ C2A_HD void begin_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   L.D = tri_distance(Rrel, Trel, ta, tb, p, q, nullptr);
   if (L.mode == MODE_DISTANCE) {   // C2A_Distance's first bound (C2A_PQP.cpp:998-1005)
     for (int i = 0; i < 3; i++) { L.p1[i] = p[i]; L.p2[i] = q[i]; }
-    if (L.overlap_only) {   // C2A_Collide tests the root pair too (C2A_PQP.cpp:1068)
-      double Rr[9], Tr[3], d, mt;
-      for (int i = 0; i < 9; i++) Rr[i] = g.stack[0].R[i];
-      for (int i = 0; i < 3; i++) Tr[i] = g.stack[0].T[i];
-      (void)pair_values(nodes_of(M1), nodes_of(M2), 0, 0, Rr, Tr, g.r1, g.mo[0], g.mo[1], true,
-                        d, mt);
-      if (d != 0.0) L.sp = 0;
-    }
     L.state = ST_CONTACT_WALK;
     return;
   }
@@ -1757,7 +1775,7 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
       g.mesh[1] = c.mesh[1];
       g.md[0] = a.meshes[c.mesh[0]];
       g.md[1] = a.meshes[c.mesh[1]];
-      g.contact = 1;
+      g.contact = (a.mode == MODE_DISTANCE && c.overlap_only) ? 2 : 1;
     }
     lane_sync();
     L.mode = a.mode;
@@ -1765,7 +1783,6 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     L.num_contact = 0;
     L.rel_err = c.rel_err;
     L.abs_err = c.abs_err;
-    L.overlap_only = c.overlap_only;
     L.seed1 = c.seed[0];
     L.seed2 = c.seed[1];
     L.last1 = c.seed[0];
@@ -1776,6 +1793,9 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     for (int i = 0; i < 9; i++) { R1[i] = c.R1[i]; R2[i] = c.R2[i]; }
     for (int i = 0; i < 3; i++) { T1[i] = c.T1[i]; T2[i] = c.T2[i]; }
     start_traversal(g, L, M1, M2, R1, T1, R2, T2, -1, lane);
+    // C2A_Collide tests the root pair too (C2A_PQP.cpp:1068); the host made
+    // that test when it set up the query
+    if (c.overlap_only == 2) L.sp = 0;
     // a distance query takes its first bound in begin_step
     L.state = a.mode == MODE_DISTANCE ? ST_STEP : ST_CONTACT_WALK;
     return true;

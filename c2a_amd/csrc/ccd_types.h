@@ -47,7 +47,8 @@ struct ContactDev {
   double threshold;
   int32_t mesh[2];
   int32_t seed[2];
-  int32_t overlap_only;   // C2A_Collide: enter only pairs whose RSS overlap
+  int32_t overlap_only;   // C2A_Collide: enter only pairs whose RSS overlap;
+                          // 2 = the root volumes are already disjoint
   int32_t pad;
   double rel_err, abs_err;
 };

@@ -198,6 +198,9 @@ extern "C" int emu_distance(const double* tris1, int64_t n1, const double* tris2
     c.rel_err = rel_err;
     c.abs_err = abs_err;
     c.overlap_only = overlap_only;
+    if (overlap_only &&
+        root_rss_distance(b1.nodes.data(), b2.nodes.data(), c.R1, c.T1, c.R2, c.T2) != 0.0)
+      c.overlap_only = 2;
   }
   int64_t counter = 0;
   KArgs a;
