@@ -38,7 +38,7 @@ class Params(C.Structure):
                 ("waves_per_query", C.c_int32), ("replay_log", C.c_int32), ("order", C.c_int32),
                 ("scorer", C.c_void_p), ("sample_rate", C.c_double), ("sample_cap", C.c_int64),
                 ("miss_depth", C.c_int32), ("ext_levels", C.c_int32),
-                ("translation", C.c_int32), ("pad", C.c_int32)]
+                ("translation", C.c_int32), ("strict", C.c_int32)]
 
 
 class Result(C.Structure):

@@ -7,6 +7,7 @@ Flags that matter for results:
                       no packed FP32 instructions are generated.
 """
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -17,8 +18,10 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libc2a_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["ccd_kernel.hip", "capi.cpp", "scorer.hip"]
+HIP_SOURCES = ["ccd_dispatch.cpp", "capi.cpp", "scorer.hip"]
 CXX_SOURCES = ["bvh.cpp", "compat.cpp"]
+# ccd_kernel.hip is compiled once per kernel shape listed in ccd_shapes.h
+SHAPE_SOURCE = "ccd_kernel.hip"
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",
           "-Wno-unused-function", "-Wno-unused-variable"]
@@ -38,11 +41,28 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=True, jobs=4):
+def kernel_shapes():
+    """(K, CAP, NW, LRN) of every shape in ccd_shapes.h's C2A_SHAPES list."""
+    text = open(os.path.join(CSRC, "ccd_shapes.h")).read()
+    body = text[text.index("#define C2A_SHAPES(X)"):]
+    body = body[:body.index("\n\n")]
+    return [tuple(int(v) for v in m)
+            for m in re.findall(r"X\((\d+),\s*(\d+),\s*(\d+),\s*(\d+)\)", body)]
+
+
+def build(verbose=True, jobs=None):
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)
     cc = hipcc()
+    jobs = jobs or max(1, min(16, os.cpu_count() or 4))
     procs, objs = [], []
+    for k, cap, nw, lrn in kernel_shapes():
+        obj = os.path.join(OBJDIR, "ccd_kernel_k%d_cap%d_nw%d_lrn%d.o" % (k, cap, nw, lrn))
+        cmd = [cc, "-x", "hip", "-c", os.path.join(CSRC, SHAPE_SOURCE), "-o", obj,
+               "-DC2A_SK=%d" % k, "-DC2A_SCAP=%d" % cap, "-DC2A_SNW=%d" % nw,
+               "-DC2A_SLRN=%d" % lrn] + COMMON + HIP_FLAGS
+        objs.append(obj)
+        procs.append(cmd)
     for src in HIP_SOURCES:
         obj = os.path.join(OBJDIR, src + ".o")
         cmd = [cc, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + HIP_FLAGS

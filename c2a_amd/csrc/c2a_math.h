@@ -181,12 +181,17 @@ struct Motion {
   double axis[3];
   double angVel;
   double pad;
+  // The axis the motion bounds cross with world-frame directions: `axis`
+  // itself (the reference, InterpMotion.cpp:838-845, 784-791) or, in strict
+  // mode, the world-frame axis R0 * axis (q(t) = q0 * q_rel(t) turns about
+  // the body-frame `axis`, i.e. about R0 * axis in world coordinates).
+  double bax[3];
 };
 
 // CInterpMotion_Linear ctor + velocity() + LinearAngularVelocity
 // (InterpMotion.cpp:148-168, 486-491, 228-270).  Host only (acos).
 inline void motion_init(Motion* m, const double* R0, const double* T0, const double* R1,
-                        const double* T1) {
+                        const double* T1, bool strict = false) {
   for (int i = 0; i < 9; i++) m->R0[i] = R0[i];
   for (int i = 0; i < 3; i++) m->T0[i] = T0[i];
   sub3(m->cv, T1, T0);
@@ -211,6 +216,8 @@ inline void motion_init(Motion* m, const double* R0, const double* T0, const dou
     m->axis[0] = tg[0] * inv; m->axis[1] = tg[1] * inv; m->axis[2] = tg[2] * inv;
   }
   m->pad = 0.0;
+  if (strict) mxv(m->bax, m->R0, m->axis);
+  else for (int i = 0; i < 3; i++) m->bax[i] = m->axis[i];
 }
 
 // CInterpMotion_Linear::integrate (InterpMotion.cpp:516-568): pose at t
@@ -231,11 +238,19 @@ C2A_HD void motion_integrate(const Motion& m, double t, double* R, double* T, do
   if (q_out) { q_out[0] = q[0]; q_out[1] = q[1]; q_out[2] = q[2]; q_out[3] = q[3]; }
 }
 
+// Admissible time a tested leaf pair contributes to mint.  Reference: only
+// an improving leaf (dTri <= distance, C2A.cpp:1150) contributes, and a
+// touching one (dTri == 0: zero-length normalise, NaN) does not.  Strict:
+// every tested leaf contributes, a touching one with 0.
+C2A_HD double leaf_time(double dTri, double mleaf, int strict) {
+  return (strict && dTri == 0.0) ? 0.0 : mleaf;
+}
+
 // computeTOC_MotionBound (InterpMotion.cpp:831-881); N is normalised in place.
 C2A_HD double motion_bound(const Motion& m, double angR, double* N) {
   double cr[3];
   normalize3(N);
-  cross3(cr, m.axis, N);
+  cross3(cr, m.bax, N);
   double w = angR * len3(cr) * m.angVel;
   double v = dot3(m.cv, N);
   if (v < 0) v = 0;
@@ -253,7 +268,7 @@ C2A_HD double motion_bound_leaf(const Motion& m, double angR, double* S) {
     v = dot3(m.cv, S);
     if (v < 0) v = 0;
   } else {
-    double cwc[3] = {m.axis[0] * m.angVel, m.axis[1] * m.angVel, m.axis[2] * m.angVel};
+    double cwc[3] = {m.bax[0] * m.angVel, m.bax[1] * m.angVel, m.bax[2] * m.angVel};
     double cr[3];
     cross3(cr, cwc, S);
     w = angR * len3(cr);

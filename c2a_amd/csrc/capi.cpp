@@ -185,7 +185,7 @@ int grid_blocks(int K, int CAP, int NW, int64_t nq, bool learned = false) {
 }
 
 int prepare_queries(const std::vector<MeshDev>& tab, const c2a_query* q, int64_t nq,
-                    std::vector<QueryDev>& out, std::vector<int32_t>& pairs) {
+                    bool strict, std::vector<QueryDev>& out, std::vector<int32_t>& pairs) {
   out.resize((size_t)nq);
   pairs.resize((size_t)nq * 2);
   for (int64_t i = 0; i < nq; i++) {
@@ -202,7 +202,7 @@ int prepare_queries(const std::vector<MeshDev>& tab, const c2a_query* q, int64_t
       double R0[9], T0[3], R1[9], T1[3];
       unpack_pose(x.pose[2 * k], R0, T0);
       unpack_pose(x.pose[2 * k + 1], R1, T1);
-      motion_init(&d.mo[k], R0, T0, R1, T1);
+      motion_init(&d.mo[k], R0, T0, R1, T1, strict);
     }
   }
   return C2A_OK;
@@ -231,7 +231,8 @@ int fill_args(c2a_batch* b, const c2a_params* p) {
   a.feats = (b->scorer || b->sample_cap > 0) ? 1 : 0;
   a.miss_depth = p->miss_depth < 0 ? 0 : p->miss_depth;
   a.ext_levels = p->ext_levels < 0 ? 0 : (p->ext_levels > 8 ? 8 : p->ext_levels);
-  a.translation = p->translation != 0 ? 1 : 0;
+  a.strict = p->strict != 0 ? 1 : 0;
+  a.translation = (p->translation != 0 && !a.strict) ? 1 : 0;
   if (b->sample_cap > 0) {
     const double r = p->sample_rate < 0 ? 0.0 : (p->sample_rate > 1 ? 1.0 : p->sample_rate);
     a.sample_thr = (uint64_t)(r * 4294967296.0);
@@ -277,7 +278,7 @@ void c2a_params_default(c2a_params* p) {
   p->miss_depth = 2;
   p->ext_levels = 0;
   p->translation = 1;
-  p->pad = 0;
+  p->strict = 0;
 }
 
 int c2a_mesh_create(const double* tri_xyz9, const int32_t* tri_ids, const int32_t* vert_idx3,
@@ -356,9 +357,13 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
   if (!b) return fail(C2A_ERR_OOM, "host allocation failed");
   std::vector<QueryDev> hq;
   std::vector<int32_t> pairs;
+  if (params && params->strict != 0 && params->strict != 1) {
+    delete b;
+    return fail(C2A_ERR_BAD_ARG, "strict must be 0 or 1");
+  }
   try {
     if ((rc = make_mesh_table(meshes, nmesh, b->mtab)) ||
-        (rc = prepare_queries(b->mtab, q, nq, hq, pairs))) {
+        (rc = prepare_queries(b->mtab, q, nq, params && params->strict, hq, pairs))) {
       delete b;
       return rc;
     }
@@ -715,6 +720,7 @@ int c2a_motion_prepare(const double* pose_start, const double* pose_end, double*
 int c2a_motion_pose(const double* motion24, double t, double* pose12, double* quat4) {
   if (!motion24 || !pose12) return fail(C2A_ERR_BAD_ARG, "null argument");
   Motion m;
+  memset(&m, 0, sizeof(m));
   memcpy(&m, motion24, sizeof(double) * 24);
   motion_integrate(m, t, pose12, pose12 + 9, quat4);
   return C2A_OK;

@@ -1219,8 +1219,11 @@ C2A_HD int scan(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
     const int stop = stopm ? (int)__builtin_ctzll(stopm) : n;
     const bool in = active && j < stop;
     const bool expand = in && internal && desc;
-    // pruned decisions lower mint (C2A.cpp:1284-1349, `else if`)
-    const double mc = wave_mind((in && !desc) ? E.mt : inf);
+    // pruned decisions lower mint (C2A.cpp:1284-1349, `else if`); in strict
+    // mode so do the tested leaves before the stop (none of them improves)
+    const double mc = wave_mind((in && !desc)                ? E.mt
+                                : (a.strict && in && tested) ? leaf_time(E.dTri, E.mleaf, 1)
+                                                             : inf);
     if (mc < L.M) L.M = mc;
     const uint64_t em = wave_ballot(expand);
     const int nexp = (int)__builtin_popcountll(em);
@@ -1265,7 +1268,8 @@ C2A_HD int scan(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
         // improving leaf (C2A.cpp:1141-1183)
         L.D = sdt;
         L.pq_pending = 1;
-        if (sml <= L.M) L.M = sml;
+        const double ml = leaf_time(sdt, sml, a.strict);
+        if (ml <= L.M) L.M = ml;
         L.last1 = sb1;
         L.last2 = sb2;
         L.tri++;
@@ -1391,10 +1395,12 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
       if (flags & 8) {
         // replayed leaf pair (C2A.cpp:1141-1183); p1/p2 are recomputed at the
         // end of the step if this stays the last improving leaf
-        if (c.dTri <= L.D) {
+        const bool imp = c.dTri <= L.D;
+        const double ml = leaf_time(c.dTri, c.mleaf, a.strict);
+        if ((imp || a.strict) && ml <= L.M) L.M = ml;
+        if (imp) {
           L.D = c.dTri;
           L.pq_pending = 1;
-          if (c.mleaf <= L.M) L.M = c.mleaf;
           L.last1 = c.b1;
           L.last2 = c.b2;
         }
@@ -1410,12 +1416,13 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
       const Slot& sl = g.slot[s];
       if (!contact) {
         const double dTri = unid(sl.dTri);
-        if (dTri <= L.D) {
+        const bool imp = dTri <= L.D;
+        const double ml = leaf_time(dTri, unid(sl.mleaf), a.strict);
+        if ((imp || a.strict) && ml <= L.M) L.M = ml;
+        if (imp) {
           L.D = dTri;
           L.pq_pending = 0;
           for (int i = 0; i < 3; i++) { L.p1[i] = unid(sl.p[i]); L.p2[i] = unid(sl.q[i]); }
-          const double ml = unid(sl.mleaf);
-          if (ml <= L.M) L.M = ml;
           L.last1 = uni(sl.t1);
           L.last2 = uni(sl.t2);
         }

@@ -1,4 +1,6 @@
 // ccd_kernel.hip -- batched conservative-advancement CCD on gfx950.
+// Compiled once per kernel shape (ccd_shapes.h); the dispatcher is
+// ccd_dispatch.cpp.
 //
 // One workgroup of NW wavefronts per query.  The kernel is persistent:
 // workgroups take queries from a device work queue until it drains.  Each
@@ -12,8 +14,8 @@
 
 #include "c2a_math.h"
 #include "bvh.h"
-#include "ccd_kernel.h"
 #include "ccd_core.h"
+#include "ccd_shapes.h"
 #include "mlp_dev.h"
 
 namespace c2a {
@@ -58,10 +60,10 @@ __global__ void __launch_bounds__(64 * NW) ccd_kernel(KArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// host-side launch
+// host-side launch of this unit's shape
 // ---------------------------------------------------------------------------
 template <int K, int CAP, int NW, bool LRN>
-static hipError_t launch_one(const KArgs& a, int blocks, hipStream_t s) {
+hipError_t launch_shape(const KArgs& a, int blocks, hipStream_t s) {
   const size_t lds = sizeof(Wave<K, CAP>);
   auto fn = ccd_kernel<K, CAP, NW, LRN>;
   hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -72,7 +74,7 @@ static hipError_t launch_one(const KArgs& a, int blocks, hipStream_t s) {
 }
 
 template <int K, int CAP, int NW, bool LRN>
-static int occupancy_one() {
+int occupancy_shape() {
   const size_t lds = sizeof(Wave<K, CAP>);
   auto fn = ccd_kernel<K, CAP, NW, LRN>;
   if (hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
@@ -83,49 +85,11 @@ static int occupancy_one() {
   return n;
 }
 
-// Instantiated shapes: every K with one wavefront per query.  The kernel
-// supports NW > 1 (the replay then uses every wavefront), but a second
-// instantiation of the same K/CAP makes the compiler outline the shared
-// per-query code, which costs more than the extra wavefronts gain.
-// The learned order (LRN) is instantiated at the default depth only.
-#define C2A_SHAPES(X)                                                                  \
-  X(1, 48, 1, 0) X(2, 48, 1, 0) X(3, 48, 1, 0) X(4, 48, 1, 0) X(5, 48, 1, 0)            \
-  X(6, 48, 1, 0) X(1, 96, 1, 0) X(2, 96, 1, 0) X(3, 96, 1, 0) X(4, 96, 1, 0)            \
-  X(5, 96, 1, 0) X(6, 96, 1, 0) X(5, 48, 1, 1) X(5, 96, 1, 1)
-
-bool kernel_shape_ok(int K, int CAP, int NW, bool LRN) {
-#define C2A_OK_CASE(k, cap, nw, lrn) \
-  if (K == k && CAP == cap && NW == nw && LRN == (lrn != 0)) return true;
-  C2A_SHAPES(C2A_OK_CASE)
-#undef C2A_OK_CASE
-  return false;
-}
-
-size_t kernel_lds_bytes(int K, int CAP) {
-#define C2A_LDS_CASE(k, cap, nw, lrn) \
-  if (K == k && CAP == cap) return sizeof(Wave<k, cap>);
-  C2A_SHAPES(C2A_LDS_CASE)
-#undef C2A_LDS_CASE
-  return 0;
-}
-
-int kernel_blocks_per_cu(int K, int CAP, int NW, bool LRN) {
-#define C2A_OCC_CASE(k, cap, nw, lrn) \
-  if (K == k && CAP == cap && NW == nw && LRN == (lrn != 0)) \
-    return occupancy_one<k, cap, nw, (lrn != 0)>();
-  C2A_SHAPES(C2A_OCC_CASE)
-#undef C2A_OCC_CASE
-  return 0;
-}
-
-hipError_t launch_ccd(const KArgs& a, int K, int CAP, int NW, int blocks, hipStream_t s) {
-  const bool LRN = a.scorer.on != 0;
-#define C2A_LAUNCH_CASE(k, cap, nw, lrn)                       \
-  if (K == k && CAP == cap && NW == nw && LRN == (lrn != 0)) \
-    return launch_one<k, cap, nw, (lrn != 0)>(a, blocks, s);
-  C2A_SHAPES(C2A_LAUNCH_CASE)
-#undef C2A_LAUNCH_CASE
-  return hipErrorInvalidValue;
-}
+#if !defined(C2A_SK) || !defined(C2A_SCAP) || !defined(C2A_SNW) || !defined(C2A_SLRN)
+#error "ccd_kernel.hip is compiled once per shape: define C2A_SK, C2A_SCAP, C2A_SNW, C2A_SLRN"
+#endif
+template hipError_t launch_shape<C2A_SK, C2A_SCAP, C2A_SNW, (C2A_SLRN != 0)>(const KArgs&, int,
+                                                                            hipStream_t);
+template int occupancy_shape<C2A_SK, C2A_SCAP, C2A_SNW, (C2A_SLRN != 0)>();
 
 }  // namespace c2a

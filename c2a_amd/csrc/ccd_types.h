@@ -180,6 +180,8 @@ struct KArgs {
                            // pairs that newly pass the descend test
   int32_t translation;     // C2A_Solve's translation-only branch for queries
                            // whose two rotations are below 1e-8
+  int32_t strict;          // strict (conservative) mode: every tested leaf
+                           // pair bounds mint (motions carry the world axis)
   ScorerDev scorer;        // learned child order when scorer.on
   // training samples of replayed pairs: a pair is logged when the top 32
   // bits of mix64(key) are below sample_thr (2^32 = every pair)

@@ -92,7 +92,18 @@ typedef struct {
                                traversal, toc = mint, object 2's velocity
                                ignored); 0: the rotation path, as a direct
                                C2A_QueryTimeOfContact call does */
-  int32_t pad;
+  int32_t strict;           /* 0 (default): the reference's semantics, bit for
+                               bit.  1: conservative mode -- the motion bounds
+                               cross the world-frame rotation axis (the
+                               reference uses the body-frame axis,
+                               InterpMotion.cpp:784-791, 838-845, with world
+                               directions, C2A.cpp:1154-1164, 1240-1242),
+                               every tested leaf pair bounds mint (the
+                               reference: only improving ones, C2A.cpp:1150),
+                               a touching leaf pair bounds it by 0, and
+                               translation-only queries take the rotation
+                               path (the translation branch ignores object
+                               2's velocity, C2A.cpp:1472-1474) */
 } c2a_params;
 
 enum { C2A_ORDER_PLAIN = 0, C2A_ORDER_LEARNED = 1 };

@@ -230,7 +230,19 @@ void c2o_motion_init(c2o_motion *m, const double R0[3][3], const double T0[3],
     double inv = 1.0 / sqrt(len);
     m->axis[0] = tan3[0] * inv; m->axis[1] = tan3[1] * inv; m->axis[2] = tan3[2] * inv;
   }
+  /* The rotation is q0 * q_rel(t): q_rel turns about `axis` in the body
+   * frame, so in world coordinates the object turns about R0 * axis at every
+   * t (R(t) * axis = R0 * axis).  The strict bounds cross this world axis
+   * with the world-frame directions they are given. */
+  mxv(m->axw, m->R0, m->axis);
 }
+
+/* Strict (conservative) mode: the motion bounds use the world-frame axis,
+ * every tested leaf pair bounds mint (not only the improving ones), and a
+ * touching leaf pair (distance 0) bounds it by 0.  Off = the reference. */
+static int g_strict = 0;
+void c2o_set_strict(int on) { g_strict = on; }
+int c2o_get_strict(void) { return g_strict; }
 
 /* CInterpMotion_Linear::integrate (InterpMotion.cpp:516-568) with
  * AbsoluteRt/DeltaRt (:273-287).  qua7 (optional) = (w,x,y,z,tx,ty,tz). */
@@ -262,7 +274,7 @@ void c2o_motion_integrate(const c2o_motion *m, double t, double R[3][3],
 double c2o_motion_bound(const c2o_motion *m, double angR, const double N[3]) {
   double n[3] = {N[0], N[1], N[2]}, cr[3];
   normalize3(n);
-  cross3(cr, m->axis, n);
+  cross3(cr, g_strict ? m->axw : m->axis, n);
   double w = angR * len3(cr) * m->angVel;
   double v = dot3(m->cv, n);
   if (v < 0) v = 0;
@@ -281,7 +293,8 @@ double c2o_motion_bound_leaf(const c2o_motion *m, double angR, const double S[3]
     v = dot3(m->cv, n);
     if (v < 0) v = 0;
   } else {
-    double cwc[3] = {m->axis[0] * m->angVel, m->axis[1] * m->angVel, m->axis[2] * m->angVel};
+    const double *ax = g_strict ? m->axw : m->axis;
+    double cwc[3] = {ax[0] * m->angVel, ax[1] * m->angVel, ax[2] * m->angVel};
     double cr[3];
     cross3(cr, cwc, n);
     w = angR * len3(cr);
@@ -1128,8 +1141,8 @@ static void toc_recurse(step_ctx *cx, const m3 R, const double T[3], int b1, int
     int t1 = -B1->first_child - 1, t2 = -B2->first_child - 1;
     double dTri = tri_distance(cx->Rrel, cx->Trel, &cx->m1->tris[t1],
                                &cx->m2->tris[t2], p, q, NULL, NULL, NULL, NULL);
-    if (dTri <= res->distance) {
-      res->distance = dTri;
+    const int improving = dTri <= res->distance;
+    if (improving || g_strict) {
       double tmp1[3], tmp2[3], w1[3], w2[3], S1[3], S2[3];
       mxv(tmp1, cx->r1, p);
       add3(w1, tmp1, cx->tt1);
@@ -1137,15 +1150,20 @@ static void toc_recurse(step_ctx *cx, const m3 R, const double T[3], int b1, int
       add3(w2, tmp2, cx->tt1);
       sub3(S1, w2, w1);
       scale3(S2, S1, -1);
-      memcpy(res->p1, p, sizeof(p));
-      memcpy(res->p2, q, sizeof(q));
       double mb1 = c2o_motion_bound_leaf(cx->mo1, B1->angR, S1);
       double mb2 = c2o_motion_bound_leaf(cx->mo2, B2->angR, S2);
       double mt = dTri / (mb1 + mb2);
       if (mt < 0.0) mt = 0.0;
+      /* strict: a touching pair has no direction (0/NaN); it bounds mint by 0 */
+      if (g_strict && dTri == 0.0) mt = 0.0;
       if (mt <= res->mint) res->mint = mt;
-      cx->last1 = t1;
-      cx->last2 = t2;
+      if (improving) {
+        res->distance = dTri;
+        memcpy(res->p1, p, sizeof(p));
+        memcpy(res->p2, q, sizeof(q));
+        cx->last1 = t1;
+        cx->last2 = t2;
+      }
     }
     res->num_tri_tests++;
     return;

@@ -71,6 +71,7 @@ typedef struct {
   double cv[3];         /* T1 - T0 */
   double axis[3];       /* body-frame unit axis */
   double angVel;        /* rotation angle over t in [0,1] */
+  double axw[3];        /* world-frame axis R0 * axis (strict bounds) */
 } c2o_motion;
 
 /* Per-query result, mirroring C2A_TimeOfContactResult (C2A.h:161-224). */
@@ -109,6 +110,15 @@ typedef struct {
 /* sin/cos used by integrate: 0 = the hand-written reduction + polynomial
  * shared with the GPU path (default), 1 = libm. */
 void c2o_set_libm_trig(int use_libm);
+/* Strict (conservative) mode, off by default (= the reference): world-frame
+ * rotation axis in both motion bounds (the reference crosses the body-frame
+ * axis with world-frame directions, InterpMotion.cpp:784-791, 838-845 vs
+ * C2A.cpp:1154-1164, 1240-1242), every tested leaf pair bounds mint (the
+ * reference takes only pairs with dTri <= distance, C2A.cpp:1150), and a
+ * touching leaf pair bounds it by 0 (the reference's zero-length normalise
+ * gives NaN and no update). */
+void c2o_set_strict(int on);
+int c2o_get_strict(void);
 void c2o_sincos(double x, double *s, double *c);
 
 /* ---- primitives (exported for the pinning tests) ---- */

@@ -34,7 +34,7 @@ class Model(C.Structure):
 
 class Motion(C.Structure):
     _fields_ = [("R0", m33), ("T0", d3), ("R1", m33), ("T1", d3), ("q0", C.c_double * 4),
-                ("cv", d3), ("axis", d3), ("angVel", C.c_double)]
+                ("cv", d3), ("axis", d3), ("angVel", C.c_double), ("axw", d3)]
 
 
 class Result(C.Structure):
@@ -87,6 +87,8 @@ def lib():
         L.c2o_sincos.argtypes = [C.c_double, dp, dp]
         L.c2o_set_libm_trig.argtypes = [C.c_int]
         L.c2o_set_solve_translation.argtypes = [C.c_int]
+        L.c2o_set_strict.argtypes = [C.c_int]
+        L.c2o_get_strict.restype = C.c_int
         L.c2o_model_build.restype = C.c_int
         L.c2o_model_build.argtypes = [C.POINTER(Model), C.c_int, dp, ip, ip]
         L.c2o_model_free.argtypes = [C.POINTER(Model)]
@@ -179,6 +181,29 @@ def set_solve_translation(flag):
     """C2A_Solve's translation-only branch for queries whose two rotations are
     below 1e-8 (default on)."""
     lib().c2o_set_solve_translation(int(flag))
+
+
+def set_strict(flag):
+    """Strict (conservative) CA mode: world-frame axis in the motion bounds,
+    every tested leaf pair bounds mint, touching leaves bound it by 0.
+    Off (default) = the reference's semantics."""
+    lib().c2o_set_strict(int(flag))
+
+
+class strict_mode:
+    """Context manager: the oracle in strict mode inside the block."""
+
+    def __init__(self, on=True):
+        self.on = on
+
+    def __enter__(self):
+        self.prev = lib().c2o_get_strict()
+        set_strict(self.on)
+        return self
+
+    def __exit__(self, *exc):
+        set_strict(self.prev)
+        return False
 
 
 def jacobi_eigen(A):

@@ -124,7 +124,7 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
       double R0[9], T0[3], R1[9], T1[3];
       memcpy(R0, q[i].pose[2 * k], 72); memcpy(T0, q[i].pose[2 * k] + 9, 24);
       memcpy(R1, q[i].pose[2 * k + 1], 72); memcpy(T1, q[i].pose[2 * k + 1] + 9, 24);
-      motion_init(&hq[i].mo[k], R0, T0, R1, T1);
+      motion_init(&hq[i].mo[k], R0, T0, R1, T1, p->strict != 0);
     }
   }
   int64_t counter = 0;
@@ -150,7 +150,8 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
   a.arena_rounds = emu_arena_rounds;
   a.miss_depth = p->miss_depth;
   a.ext_levels = p->ext_levels;
-  a.translation = p->translation;
+  a.strict = p->strict != 0;
+  a.translation = p->translation && !a.strict;
   a.scorer = g_scorer;
   if (!g_keys.empty() && a.logcap > 0) {
     a.sample_thr = (uint64_t)((g_rate < 0 ? 0 : g_rate > 1 ? 1 : g_rate) * 4294967296.0);
