@@ -1509,71 +1509,133 @@ static double query_toc_translation(c2o_model *m1, c2o_model *m2, c2o_motion *mo
   return res->toc;
 }
 
+/* The CA loop of C2A_QueryTimeOfContact, rotation path (C2A.cpp:2029-2143),
+ * over an abstract step: step(ctx, lamda, res) runs C2A_TimeOfContactStep at
+ * time lamda (numCA, mint and UpboundTOC as the loop leaves them in res) and
+ * sets res->distance and res->mint.  The first step runs at lamda 0.  Leaves
+ * toc, collisionfree, numCA and the iteration-cap flag in res; returns the
+ * TOC lamda at which R_toc/T_toc must be evaluated, or -1 when collision
+ * free. */
+typedef void (*ca_step_fn)(void *ctx, double lamda, c2o_result *res);
+
+static double ca_loop(ca_step_fn step, void *ctx, c2o_result *res, double tol_d,
+                      double tol_t, int max_iter) {
+  double lamda = 0.0, lastLamda, dist, mint;
+  int nItrs = 0;
+  step(ctx, 0.0, res);
+  dist = res->distance;
+  mint = res->mint;
+  res->numCA = 1;
+  lastLamda = mint;                          /* C2A.cpp:2072 */
+  while (dist > tol_d) {
+    nItrs++;
+    if (nItrs > max_iter) { res->flags |= C2O_FLAG_ITER_CAP; break; }
+    if (mint >= 1.0) {                       /* C2A.cpp:2086-2093 */
+      res->collisionfree = 1;
+      res->toc = 0;
+      return -1.0;
+    }
+    double dlamda = mint;
+    if (dlamda < tol_t) break;               /* C2A.cpp:2098-2101 */
+    lamda += dlamda;
+    if (lamda >= 1.0) {                      /* C2A.cpp:2103-2110 */
+      res->collisionfree = 1;
+      res->toc = 0;
+      return -1.0;
+    }
+    lastLamda = lamda;
+    res->numCA++;
+    res->UpboundTOC = 1.0 - lamda;
+    step(ctx, lamda, res);
+    dist = res->distance;
+    mint = res->mint;
+  }
+  /* the dist == 0 && mint < 0 branch (C2A.cpp:2125-2128) cannot run: mint is
+   * clamped at 0 */
+  double toc = lastLamda;
+  res->collisionfree = 0;
+  if (toc >= 1 - tol_t) toc = 0;             /* C2A.cpp:2136-2137 */
+  res->toc = toc;
+  return toc;
+}
+
+typedef struct {
+  c2o_model *m1, *m2;
+  c2o_motion *mo1, *mo2;
+  int calls;
+} query_ctx;
+
+static void query_step(void *vctx, double lamda, c2o_result *res) {
+  query_ctx *cx = (query_ctx *)vctx;
+  m3 R1, R2;
+  double T1[3], T2[3];
+  if (cx->calls++ == 0) {                    /* the first step: the start poses as given */
+    memcpy(R1, cx->mo1->R0, sizeof(m3));
+    memcpy(T1, cx->mo1->T0, sizeof(T1));
+    memcpy(R2, cx->mo2->R0, sizeof(m3));
+    memcpy(T2, cx->mo2->T0, sizeof(T2));
+  } else {
+    c2o_motion_integrate(cx->mo1, lamda, R1, T1, NULL);
+    c2o_motion_integrate(cx->mo2, lamda, R2, T2, NULL);
+  }
+  c2o_toc_step(cx->m1, cx->m2, cx->mo1, cx->mo2, R1, T1, R2, T2, cx->m1, cx->m2, res);
+}
+
 /* C2A_QueryTimeOfContact (C2A.cpp:1987-2146); with both rotations below
  * 1e-8 and the translation switch on (C2A_Solve's b_TanslationCCD), the
  * translation-only branch. */
 double c2o_query_toc(c2o_model *m1, c2o_model *m2, c2o_motion *mo1,
                      c2o_motion *mo2, c2o_result *res, double tol_d,
                      double tol_t, int max_iter) {
-  m3 R1, R2;
-  double T1[3], T2[3];
   res->num_bv_tests = 0;
   res->num_tri_tests = 0;
   res->num_contact = 0;
   res->UpboundTOC = 1;
   res->numCA = 0;
   res->flags = 0;
-  memcpy(R1, mo1->R0, sizeof(m3));
-  memcpy(T1, mo1->T0, sizeof(T1));
-  memcpy(R2, mo2->R0, sizeof(m3));
-  memcpy(T2, mo2->T0, sizeof(T2));
-  double lamda = 0.0, lastLamda = 0.0, dist, mint;
-  int nItrs = 0;
   if (mo1->angVel < 1e-8 && mo2->angVel < 1e-8) res->flags |= C2O_FLAG_TRANSLATION_ONLY;
-  if (g_solve_translation && (res->flags & C2O_FLAG_TRANSLATION_ONLY))
+  if (g_solve_translation && !g_strict && (res->flags & C2O_FLAG_TRANSLATION_ONLY))
     return query_toc_translation(m1, m2, mo1, mo2, res, tol_d);
-
-  c2o_toc_step(m1, m2, mo1, mo2, R1, T1, R2, T2, m1, m2, res);
-  dist = res->distance;
-  mint = res->mint;
-  res->numCA = 1;
-  lastLamda = mint;
-  while (dist > tol_d) {
-    nItrs++;
-    if (nItrs > max_iter) { res->flags |= C2O_FLAG_ITER_CAP; break; }
-    if (mint >= 1.0) {
-      res->collisionfree = 1;
-      res->toc = 0;
-      goto done;
-    }
-    double dlamda = mint;
-    if (dlamda < tol_t) break;
-    lamda += dlamda;
-    if (lamda >= 1.0) {
-      res->collisionfree = 1;
-      res->toc = 0;
-      goto done;
-    }
-    lastLamda = lamda;
-    res->numCA++;
-    c2o_motion_integrate(mo1, lamda, R1, T1, NULL);
-    c2o_motion_integrate(mo2, lamda, R2, T2, NULL);
-    res->UpboundTOC = 1.0 - lamda;
-    c2o_toc_step(m1, m2, mo1, mo2, R1, T1, R2, T2, m1, m2, res);
-    dist = res->distance;
-    mint = res->mint;
-  }
-  {
-    double toc = lastLamda;
-    res->collisionfree = 0;
-    if (toc >= 1 - tol_t) toc = 0;
-    res->toc = toc;
-    c2o_motion_integrate(mo1, toc, res->R_toc, res->T_toc, NULL);
-  }
-done:
+  query_ctx cx = {m1, m2, mo1, mo2, 0};
+  const double toc = ca_loop(query_step, &cx, res, tol_d, tol_t, max_iter);
+  if (toc >= 0.0) c2o_motion_integrate(mo1, toc, res->R_toc, res->T_toc, NULL);
   res->out_last_triA = m1->last_tri;
   res->out_last_triB = m2->last_tri;
   return res->toc;
+}
+
+/* The CA loop over a scripted step (tests of the loop's branches): call k
+ * (k = 0 first) sets distance = dist[k] and mint = mint[k] and records the
+ * lamda, numCA and UpboundTOC it was called with.  Calls beyond n reuse the
+ * last entry.  Returns the number of step calls. */
+typedef struct {
+  const double *dist, *mint;
+  int n, calls, cap;
+  double *lamda_out, *up_out;
+  int *numca_out;
+} scripted_ctx;
+
+static void scripted_step(void *vctx, double lamda, c2o_result *res) {
+  scripted_ctx *cx = (scripted_ctx *)vctx;
+  const int k = cx->calls < cx->n ? cx->calls : cx->n - 1;
+  if (cx->calls < cx->cap) {
+    cx->lamda_out[cx->calls] = lamda;
+    cx->up_out[cx->calls] = res->UpboundTOC;
+    cx->numca_out[cx->calls] = res->numCA;
+  }
+  cx->calls++;
+  res->distance = cx->dist[k];
+  res->mint = cx->mint[k];
+}
+
+int c2o_ca_loop_scripted(const double *dist, const double *mint, int n, double tol_d,
+                         double tol_t, int max_iter, c2o_result *res, double *lamda_out,
+                         double *up_out, int *numca_out, int cap) {
+  scripted_ctx cx = {dist, mint, n, 0, cap, lamda_out, up_out, numca_out};
+  memset(res, 0, sizeof(*res));
+  res->UpboundTOC = 1;
+  (void)ca_loop(scripted_step, &cx, res, tol_d, tol_t, max_iter);
+  return cx.calls;
 }
 
 /* ------------------------------------------------------------------------ */

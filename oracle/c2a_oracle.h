@@ -165,7 +165,18 @@ void c2o_toc_step(const c2o_model *m1, const c2o_model *m2,
 double c2o_query_toc(c2o_model *m1, c2o_model *m2, c2o_motion *mo1,
                      c2o_motion *mo2, c2o_result *res, double tol_d,
                      double tol_t, int max_iter);
-void c2o_set_solve_translation(int on);   /* default 1 (C2A_Solve semantics) */
+void c2o_set_solve_translation(int on);   /* default 1 (C2A_Solve semantics);
+                                            strict mode always takes the
+                                            rotation path */
+
+/* The CA loop (C2A.cpp:2029-2143) driven by a scripted step: the k-th step
+ * call returns distance dist[k] and mint mint[k] (the last entry repeats).
+ * Writes toc, collisionfree, numCA and flags to res, and per call (up to
+ * cap) the lamda, UpboundTOC and numCA it was made with.  Returns the
+ * number of step calls. */
+int c2o_ca_loop_scripted(const double *dist, const double *mint, int n, double tol_d,
+                         double tol_t, int max_iter, c2o_result *res, double *lamda_out,
+                         double *up_out, int *numca_out, int cap);
 
 /* Contact enumeration at absolute poses (C2A_QueryContact,
  * C2A.cpp:1937-1966 -> 1727-1775 -> 1525-1725).  Returns the count; fills

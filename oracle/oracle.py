@@ -88,6 +88,9 @@ def lib():
         L.c2o_set_libm_trig.argtypes = [C.c_int]
         L.c2o_set_solve_translation.argtypes = [C.c_int]
         L.c2o_set_strict.argtypes = [C.c_int]
+        L.c2o_ca_loop_scripted.restype = C.c_int
+        L.c2o_ca_loop_scripted.argtypes = [dp, dp, C.c_int, C.c_double, C.c_double, C.c_int,
+                                           C.POINTER(Result), dp, dp, ip, C.c_int]
         L.c2o_get_strict.restype = C.c_int
         L.c2o_model_build.restype = C.c_int
         L.c2o_model_build.argtypes = [C.POINTER(Model), C.c_int, dp, ip, ip]
@@ -181,6 +184,21 @@ def set_solve_translation(flag):
     """C2A_Solve's translation-only branch for queries whose two rotations are
     below 1e-8 (default on)."""
     lib().c2o_set_solve_translation(int(flag))
+
+
+def ca_loop_scripted(dist, mint, tol_d, tol_t, max_iter=150, cap=512):
+    """The CA loop with a scripted step (call k returns dist[k], mint[k]).
+    Returns (Result, calls, lamdas, upbounds, numCAs) of the step calls."""
+    dist, mint = f64(dist), f64(mint)
+    assert len(dist) == len(mint) > 0
+    res = Result()
+    lam, up = np.zeros(cap), np.zeros(cap)
+    nca = np.zeros(cap, dtype=np.int32)
+    n = lib().c2o_ca_loop_scripted(_dp(dist), _dp(mint), len(dist), float(tol_d), float(tol_t),
+                                   int(max_iter), C.byref(res), _dp(lam), _dp(up), _ip(nca),
+                                   cap)
+    k = min(n, cap)
+    return res, n, lam[:k], up[:k], nca[:k]
 
 
 def set_strict(flag):

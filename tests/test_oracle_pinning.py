@@ -344,14 +344,29 @@ def rotation_path():
 
 def test_ca_loop_immediate_contact_returns_first_mint(spheres, rotation_path):
     """If the first step already has dist <= tol_d the loop does not run and
-    the reported TOC is the first step's mint (C2A.cpp:2072, 2131)."""
+    the reported TOC is the first step's mint (C2A.cpp:2072, 2131): two
+    spheres 0.5 * tol_d apart (placed by bisection on the exact distance),
+    approaching, report toc == mint > 0 after one step."""
     t1, _ = spheres
     m1, m2 = O.OModel(t1), O.OModel(t1)
     I = np.eye(3)
-    res = O.query_toc(m1, m2, (I, np.zeros(3)), (I, np.zeros(3)), (I, np.zeros(3)),
-                      (I, np.array([0.5, 0, 0])), 1e-3, 1e-4)
+
+    def gap(x):
+        return O.distance(m1, m2, I, np.zeros(3), I, np.array([x, 0, 0]), seeds=(0, 0)).distance
+    lo, hi = 1.0, 3.0            # touching at lo, apart at hi
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if gap(mid) < 5e-4:
+            lo = mid
+        else:
+            hi = mid
+    x0 = hi
+    assert 4e-4 < gap(x0) <= 6e-4
+    res = O.query_toc(m1, m2, (I, np.zeros(3)), (I, np.zeros(3)), (I, np.array([x0, 0, 0])),
+                      (I, np.array([x0 - 0.5, 0, 0])), 1e-3, 1e-4)
     assert res.numCA == 1 and res.collisionfree == 0
-    assert res.toc == pytest.approx(res.mint) or res.toc == 0.0
+    assert res.distance <= 1e-3
+    assert res.mint > 0 and res.toc == res.mint
 
 
 def test_separating_motion_is_collision_free(spheres, rotation_path):
