@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libc2a_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "c2a_amd.h")
 
 OK, ERR_BAD_ARG, ERR_OOM, ERR_HIP, ERR_NO_DEVICE, ERR_UNSUPPORTED, ERR_EMPTY_MODEL = range(7)
-FLAG_ITER_CAP, FLAG_TRANSLATION_ONLY, FLAG_STACK_OVERFLOW = 1, 2, 4
+FLAG_ITER_CAP, FLAG_TRANSLATION_ONLY, FLAG_STACK_OVERFLOW, FLAG_BAD_INPUT = 1, 2, 4, 8
+NOALLOC_SCRATCH_MIB, NOALLOC_MAX_MESHES = 128, 32
 ORDER_PLAIN, ORDER_LEARNED = 0, 1
 NF = 32
 
@@ -87,7 +88,7 @@ EXPORTS = ["c2a_params_default", "c2a_mesh_create", "c2a_mesh_destroy", "c2a_mes
            "c2a_status_string", "c2a_last_error", "c2a_scorer_create", "c2a_scorer_destroy",
            "c2a_scorer_set_params", "c2a_scorer_get_params", "c2a_scorer_infer",
            "c2a_scorer_grad", "c2a_scorer_train", "c2a_scorer_save", "c2a_scorer_load",
-           "c2a_batch_samples"]
+           "c2a_batch_samples", "c2a_ccd_batch_noalloc"]
 
 
 def lib():
@@ -125,6 +126,8 @@ def lib():
     L.c2a_distance.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, ip, C.c_double, C.c_double,
                                i64, vp, vp]
     L.c2a_collide_distance.argtypes = L.c2a_distance.argtypes
+    L.c2a_ccd_batch_noalloc.argtypes = [C.POINTER(vp), C.c_int32, vp, vp, vp, i64,
+                                        C.POINTER(Params), vp, vp]
     L.c2a_motion_prepare.argtypes = [dp, dp, dp]
     L.c2a_motion_pose.argtypes = [dp, C.c_double, dp, dp]
     L.c2a_status_string.argtypes = [C.c_int]
@@ -371,6 +374,27 @@ class Scorer:
                                       int(epochs), int(batch), C.c_float(lr), C.c_uint64(seed),
                                       losses.ctypes.data_as(C.c_void_p)))
         return losses
+
+
+def ccd_batch_noalloc(meshes, motions, mesh_pairs, seed_tris, nq, params=None, out=None,
+                      stream=None):
+    """c2a_ccd_batch_noalloc: every array argument is a device address (int)
+    the caller owns -- motions nq x 2 x 24 doubles (motion_records), mesh
+    pairs and seed pairs nq x 2 int32 (seeds may be None), out nq results.
+    Returns once the work is queued on `stream`."""
+    p = params if params is not None else default_params()
+    _check(lib().c2a_ccd_batch_noalloc(_mesh_table(meshes), len(meshes), motions, mesh_pairs,
+                                       seed_tris, int(nq), C.byref(p), out, stream))
+
+
+def motion_records(P00, P01, P10, P11):
+    """(nq, 2, 24) motion records (c2a_motion_prepare) of both objects."""
+    nq = len(P00)
+    out = np.zeros((nq, 2, 24))
+    for i in range(nq):
+        out[i, 0] = motion_prepare(P00[i], P01[i])
+        out[i, 1] = motion_prepare(P10[i], P11[i])
+    return out
 
 
 def motion_prepare(pose_start, pose_end):

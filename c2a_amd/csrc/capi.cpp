@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -208,6 +209,21 @@ int prepare_queries(const std::vector<MeshDev>& tab, const c2a_query* q, int64_t
   return C2A_OK;
 }
 
+// Kernel arguments that come from the query parameters.
+void params_to_args(KArgs& a, const c2a_params* p) {
+  a.tol_d = p->tol_d;
+  a.tol_t = p->tol_t;
+  a.contact_scale = p->contact_scale;
+  a.contact_offset = p->contact_offset;
+  a.max_iter = p->max_iter;
+  a.with_contacts = p->with_contacts;
+  a.mode = MODE_QUERY;
+  a.miss_depth = p->miss_depth < 0 ? 0 : p->miss_depth;
+  a.ext_levels = p->ext_levels < 0 ? 0 : (p->ext_levels > 8 ? 8 : p->ext_levels);
+  a.strict = p->strict != 0 ? 1 : 0;
+  a.translation = (p->translation != 0 && !a.strict) ? 1 : 0;
+}
+
 int fill_args(c2a_batch* b, const c2a_params* p) {
   c2a_params def;
   c2a_params_default(&def);
@@ -219,20 +235,10 @@ int fill_args(c2a_batch* b, const c2a_params* p) {
   a.out = (c2a_result*)b->d_out.p;
   a.counter = (int64_t*)b->d_counter.p;
   a.nq = b->nq;
-  a.tol_d = p->tol_d;
-  a.tol_t = p->tol_t;
-  a.contact_scale = p->contact_scale;
-  a.contact_offset = p->contact_offset;
-  a.max_iter = p->max_iter;
-  a.with_contacts = p->with_contacts;
-  a.mode = MODE_QUERY;
+  params_to_args(a, p);
   a.logcap = b->logcap;
   a.arena = (unsigned char*)b->d_arena.p;
   a.feats = (b->scorer || b->sample_cap > 0) ? 1 : 0;
-  a.miss_depth = p->miss_depth < 0 ? 0 : p->miss_depth;
-  a.ext_levels = p->ext_levels < 0 ? 0 : (p->ext_levels > 8 ? 8 : p->ext_levels);
-  a.strict = p->strict != 0 ? 1 : 0;
-  a.translation = (p->translation != 0 && !a.strict) ? 1 : 0;
   if (b->sample_cap > 0) {
     const double r = p->sample_rate < 0 ? 0.0 : (p->sample_rate > 1 ? 1.0 : p->sample_rate);
     a.sample_thr = (uint64_t)(r * 4294967296.0);
@@ -257,7 +263,133 @@ int run_kernel(const KArgs& a, int K, int CAP, int NW, int blocks, hipStream_t s
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Allocation-free batch (c2a_ccd_batch_noalloc).  Its device memory beyond
+// the caller's inputs and outputs -- the work-queue counter, the mesh table
+// and the replay arenas -- is this file-scope array, one copy per device,
+// held until the process ends.  Host code never passes the array's name to
+// the device: each device's address comes from hipGetSymbolAddress, called
+// once per device with that device current.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchBytes = (size_t)C2A_NOALLOC_SCRATCH_MIB << 20;
+constexpr size_t kScratchTable = 256;    // mesh table offset (the counter is at 0)
+constexpr size_t kScratchArena = 1024;   // first replay arena
+constexpr int kMaxDevices = 64;
+static_assert(kScratchTable + C2A_NOALLOC_MAX_MESHES * sizeof(MeshDev) <= kScratchArena,
+              "mesh table overlaps the arenas");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+struct NoallocTable {
+  MeshDev m[C2A_NOALLOC_MAX_MESHES];
+};
+
+// The mesh table travels by value as a kernel argument (no host buffer has
+// to outlive the call) and is written into the scratch by this kernel.
+__global__ void write_mesh_table(MeshDev* dst, NoallocTable t, int n) {
+  const int i = threadIdx.x;
+  if (i < n) dst[i] = t.m[i];
+}
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kMaxDevices];   // each device's scratch address
+
+// The current device's scratch address, looked up on the first call there.
+int device_scratch(unsigned char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(C2A_ERR_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return C2A_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int c2a_ccd_batch_noalloc(c2a_mesh* const* meshes, int32_t nmesh, const double* motions,
+                          const int32_t* mesh_pairs, const int32_t* seed_tris, int64_t nq,
+                          const c2a_params* params, c2a_result* out, void* stream) {
+  if (nq < 0) return fail(C2A_ERR_BAD_ARG, "nq must be >= 0");
+  if (nq == 0) return C2A_OK;
+  if (!motions || !mesh_pairs || !out) return fail(C2A_ERR_BAD_ARG, "null device array");
+  if (!meshes || nmesh <= 0) return fail(C2A_ERR_BAD_ARG, "empty mesh table");
+  if (nmesh > C2A_NOALLOC_MAX_MESHES)
+    return fail(C2A_ERR_BAD_ARG, "the allocation-free batch takes at most " +
+                                     std::to_string(C2A_NOALLOC_MAX_MESHES) + " meshes");
+  c2a_params def;
+  c2a_params_default(&def);
+  const c2a_params* p = params ? params : &def;
+  if (p->order != C2A_ORDER_PLAIN)
+    return fail(C2A_ERR_BAD_ARG, "the allocation-free batch runs the plain order only");
+  if (p->sample_cap != 0)
+    return fail(C2A_ERR_BAD_ARG, "the allocation-free batch logs no training samples");
+  if (p->strict != 0 && p->strict != 1) return fail(C2A_ERR_BAD_ARG, "strict must be 0 or 1");
+  int K = 5, NW = 1, rc;
+  if ((rc = pick_shape(p, K, NW))) return rc;
+  if ((rc = check_device())) return rc;
+  NoallocTable tab;
+  memset(&tab, 0, sizeof(tab));
+  int maxd = 0;
+  for (int i = 0; i < nmesh; i++) {
+    const c2a_mesh* m = meshes[i];
+    if (!m) return fail(C2A_ERR_BAD_ARG, "null mesh in table");
+    if (!m->nodes.p || !m->tris.p)
+      return fail(C2A_ERR_BAD_ARG, "mesh " + std::to_string(i) +
+                                       " is not on the device yet: run it once through "
+                                       "c2a_ccd_batch or c2a_batch_create first");
+    tab.m[i].nodes = (const GpuNode*)m->nodes.p;
+    tab.m[i].tris = (const double*)m->tris.p;
+    tab.m[i].ntri = (int32_t)m->ntri;
+    tab.m[i].depth = m->host.max_depth;
+    maxd = std::max(maxd, m->host.max_depth);
+  }
+  // any two meshes of the table may meet: stack capacity from the deepest
+  const int need = 2 * maxd + 2;
+  const int CAP = need <= 48 ? 48 : (need <= 96 ? 96 : -1);
+  if (CAP < 0) return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
+  const int logcap = p->replay_log == -1 ? 1024 : p->replay_log;
+  if (logcap < 0) return fail(C2A_ERR_BAD_ARG, "replay_log must be -1 (1024) or >= 0");
+  const size_t room = kScratchBytes - kScratchArena;
+  const size_t slice = logcap > 0 ? arena_bytes(logcap) : 0;
+  if (slice > room)
+    return fail(C2A_ERR_BAD_ARG, "replay_log " + std::to_string(logcap) + " needs a " +
+                                     std::to_string(slice) + "-byte arena; the scratch holds " +
+                                     std::to_string(room) + " bytes");
+  unsigned char* sp = nullptr;
+  if ((rc = device_scratch(&sp))) return rc;
+  int blocks = grid_blocks(K, CAP, NW, nq);
+  if (slice) blocks = (int)std::min<size_t>((size_t)blocks, room / slice);
+  KArgs a;
+  memset(&a, 0, sizeof(a));
+  params_to_args(a, p);
+  a.meshes = (const MeshDev*)(sp + kScratchTable);
+  a.qmot = motions;
+  a.qpairs = mesh_pairs;
+  a.qseeds = seed_tris;
+  a.nmesh = nmesh;
+  a.out = out;
+  a.counter = (int64_t*)sp;
+  a.nq = nq;
+  a.logcap = logcap;
+  a.arena = logcap > 0 ? sp + kScratchArena : nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(write_mesh_table, dim3(1), dim3(C2A_NOALLOC_MAX_MESHES), 0, s,
+                     (MeshDev*)(sp + kScratchTable), tab, nmesh);
+  HIP_TRY(hipGetLastError());
+  return run_kernel(a, K, CAP, NW, blocks, s);
+}
 
 void c2a_params_default(c2a_params* p) {
   if (!p) return;

@@ -1807,20 +1807,59 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     L.state = a.mode == MODE_DISTANCE ? ST_STEP : ST_CONTACT_WALK;
     return true;
   }
-  const QueryDev& q = a.q[qi];
-  if (lane == 0) {
-    g.mo[0] = q.mo[0];
-    g.mo[1] = q.mo[1];
-    g.mesh[0] = q.mesh[0];
-    g.mesh[1] = q.mesh[1];
-    g.md[0] = a.meshes[q.mesh[0]];
-    g.md[1] = a.meshes[q.mesh[1]];
+  int s1, s2;
+  if (a.qmot) {
+    // device-resident records: mesh ids and seeds are checked here, since
+    // the host never sees them; a bad query is flagged and skipped
+    const int m1 = a.qpairs[2 * qi], m2 = a.qpairs[2 * qi + 1];
+    bool bad = m1 < 0 || m1 >= a.nmesh || m2 < 0 || m2 >= a.nmesh;
+    s1 = a.qseeds ? a.qseeds[2 * qi] : 0;
+    s2 = a.qseeds ? a.qseeds[2 * qi + 1] : 0;
+    s1 = s1 < 0 ? 0 : s1;
+    s2 = s2 < 0 ? 0 : s2;
+    if (!bad) bad = s1 >= a.meshes[m1].ntri || s2 >= a.meshes[m2].ntri;
+    if (bad) {
+      if (lane == 0) {
+        c2a_result* r = a.out + qi;
+        memset(r, 0, sizeof(*r));
+        r->flags = C2A_FLAG_BAD_INPUT;
+        r->last_tri[0] = r->last_tri[1] = -1;
+      }
+      L.state = ST_FETCH;
+      return true;
+    }
+    if (lane == 0) {
+      for (int k = 0; k < 2; k++) {
+        const double* src = a.qmot + (2 * qi + k) * 24;
+        double* dst = reinterpret_cast<double*>(&g.mo[k]);
+        for (int w = 0; w < 24; w++) dst[w] = src[w];
+        Motion& mo = g.mo[k];
+        if (a.strict) mxv(mo.bax, mo.R0, mo.axis);
+        else for (int i = 0; i < 3; i++) mo.bax[i] = mo.axis[i];
+      }
+      g.mesh[0] = m1;
+      g.mesh[1] = m2;
+      g.md[0] = a.meshes[m1];
+      g.md[1] = a.meshes[m2];
+    }
+  } else {
+    const QueryDev& q = a.q[qi];
+    if (lane == 0) {
+      g.mo[0] = q.mo[0];
+      g.mo[1] = q.mo[1];
+      g.mesh[0] = q.mesh[0];
+      g.mesh[1] = q.mesh[1];
+      g.md[0] = a.meshes[q.mesh[0]];
+      g.md[1] = a.meshes[q.mesh[1]];
+    }
+    s1 = q.seed[0];
+    s2 = q.seed[1];
   }
   lane_sync();
-  L.seed1 = q.seed[0];
-  L.seed2 = q.seed[1];
-  L.last1 = q.seed[0];   // model last_tri starts at the seed (demo protocol)
-  L.last2 = q.seed[1];
+  L.seed1 = s1;
+  L.seed2 = s2;
+  L.last1 = s1;   // model last_tri starts at the seed (demo protocol)
+  L.last2 = s2;
   L.mode = MODE_QUERY;
   L.nlog_prev = 0;
   L.logbuf = 0;
@@ -1830,7 +1869,8 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   L.nItrs = 0;
   L.Up = 1;
   L.M = 1;
-  if (q.mo[0].angVel < 1e-8 && q.mo[1].angVel < 1e-8) L.flags |= C2A_FLAG_TRANSLATION_ONLY;
+  if (unid(g.mo[0].angVel) < 1e-8 && unid(g.mo[1].angVel) < 1e-8)
+    L.flags |= C2A_FLAG_TRANSLATION_ONLY;
   L.state = (a.translation && (L.flags & C2A_FLAG_TRANSLATION_ONLY)) ? ST_TRANS : ST_STEP;
   return true;
 }

@@ -158,6 +158,15 @@ C2A_HD ArenaView arena_view(unsigned char* base, int logcap, bool feats = false)
 struct KArgs {
   const MeshDev* meshes;
   const QueryDev* q;
+  // queries as device-resident motion records instead of q (the
+  // allocation-free entry point): nq x 2 x 24 doubles laid out as the first
+  // 24 words of Motion, mesh pairs and seed pairs (nq x 2; seeds may be
+  // null), validated against nmesh and the meshes' triangle counts
+  const double* qmot;
+  const int32_t* qpairs;
+  const int32_t* qseeds;
+  int32_t nmesh;
+  int32_t pad0;
   const StepDev* step_in;
   c2a_result* out;
   c2a_step_result* step_out;

@@ -132,8 +132,11 @@ enum {
   C2A_FLAG_TRANSLATION_ONLY = 2,  /* both rotations below 1e-8 (the
                                      translation-only branch ran when
                                      params->translation) */
-  C2A_FLAG_STACK_OVERFLOW = 4     /* traversal deeper than the stack (never
+  C2A_FLAG_STACK_OVERFLOW = 4,    /* traversal deeper than the stack (never
                                      with capacities chosen from mesh depth) */
+  C2A_FLAG_BAD_INPUT = 8          /* c2a_ccd_batch_noalloc: the query's mesh
+                                     ids or seed triangles are out of range;
+                                     it was skipped (other fields zero) */
 };
 
 void c2a_params_default(c2a_params *p);
@@ -158,6 +161,35 @@ int c2a_mesh_export(const c2a_mesh *m, double *nodes32, double *tris9);
  * stream may be NULL (the null stream). */
 int c2a_ccd_batch(c2a_mesh *const *meshes, int32_t nmesh, const c2a_query *q, int64_t nq,
                   const c2a_params *params, c2a_result *out, void *stream);
+
+/* Allocation-free batch.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  The c2a_ccd_batch
+ * computation (plain order, params->strict honoured) with device-resident
+ * inputs and outputs; it calls no hipMalloc, hipFree or other allocating
+ * routine and returns once the work is queued on `stream` (the caller
+ * synchronises).
+ *   motions     nq x 2 x 24 doubles in device memory: per query, object 1's
+ *               then object 2's record as c2a_motion_prepare writes it
+ *   mesh_pairs  nq x 2 int32 in device memory (indices into `meshes`)
+ *   seed_tris   nq x 2 int32 in device memory, or NULL (triangle 0)
+ *   out         nq results in device memory; a query whose mesh ids or seed
+ *               triangles are out of range gets C2A_FLAG_BAD_INPUT and no
+ *               other work
+ * Every mesh must already be on the device (uploaded by an earlier
+ * c2a_ccd_batch or c2a_batch_create with it); at most C2A_NOALLOC_MAX_MESHES
+ * meshes.  The library's scratch is a fixed C2A_NOALLOC_SCRATCH_MIB MiB per
+ * device, held from the first call until the process ends: the work-queue
+ * counter, the mesh table and one replay arena per resident query
+ * (params->replay_log, -1 = 1024 expansions; the grid shrinks to the
+ * arenas that fit; a log whose single arena does not fit is rejected with
+ * C2A_ERR_BAD_ARG).  Calls that overlap on one device race on that
+ * scratch: serialise them (one stream, or events). */
+#define C2A_NOALLOC_SCRATCH_MIB 128
+#define C2A_NOALLOC_MAX_MESHES 32
+int c2a_ccd_batch_noalloc(c2a_mesh *const *meshes, int32_t nmesh, const double *motions,
+                          const int32_t *mesh_pairs, const int32_t *seed_tris, int64_t nq,
+                          const c2a_params *params, c2a_result *out, void *stream);
 
 /* Device-resident batches: inputs uploaded once, run repeatedly, results
  * fetched on demand (used to time the kernel with inputs already in HBM). */
