@@ -47,7 +47,9 @@ class Result(C.Structure):
                 ("R_toc", C.c_double * 9), ("T_toc", C.c_double * 3), ("p1", C.c_double * 3),
                 ("p2", C.c_double * 3), ("bv_tests", C.c_int64), ("tri_tests", C.c_int64),
                 ("numCA", C.c_int32), ("collisionfree", C.c_int32), ("num_contact", C.c_int32),
-                ("last_tri", C.c_int32 * 2), ("flags", C.c_int32)]
+                ("last_tri", C.c_int32 * 2), ("flags", C.c_int32), ("R", C.c_double * 9),
+                ("T", C.c_double * 3), ("abs_err", C.c_double), ("rel_err", C.c_double),
+                ("UpboundTOC", C.c_double)]
 
 
 class StepQuery(C.Structure):
@@ -62,15 +64,21 @@ class StepResult(C.Structure):
     _fields_ = [("distance", C.c_double), ("mint", C.c_double), ("abs_err", C.c_double),
                 ("rel_err", C.c_double), ("p1", C.c_double * 3), ("p2", C.c_double * 3),
                 ("bv_tests", C.c_int64), ("tri_tests", C.c_int64), ("last_tri", C.c_int32 * 2),
-                ("flags", C.c_int32), ("pad", C.c_int32)]
+                ("flags", C.c_int32), ("pad", C.c_int32), ("R", C.c_double * 9),
+                ("T", C.c_double * 3)]
 
 
 RESULT_DTYPE = np.dtype([("toc", "f8"), ("distance", "f8"), ("mint", "f8"), ("R_toc", "f8", 9),
                          ("T_toc", "f8", 3), ("p1", "f8", 3), ("p2", "f8", 3),
                          ("bv_tests", "i8"), ("tri_tests", "i8"), ("numCA", "i4"),
                          ("collisionfree", "i4"), ("num_contact", "i4"), ("last_tri", "i4", 2),
-                         ("flags", "i4")])
+                         ("flags", "i4"), ("R", "f8", 9), ("T", "f8", 3), ("abs_err", "f8"),
+                         ("rel_err", "f8"), ("UpboundTOC", "f8")])
 assert RESULT_DTYPE.itemsize == C.sizeof(Result)
+
+# c2a_contact
+CONTACT_DTYPE = np.dtype([("tri", "i4", 2), ("type", "i4", 2), ("fid", "i4", (2, 3)),
+                          ("pa", "f8", 3), ("pb", "f8", 3), ("distance", "f8")])
 
 # c2a_distance_result
 DISTANCE_DTYPE = np.dtype([("distance", "f8"), ("p1", "f8", 3), ("p2", "f8", 3),
@@ -88,7 +96,8 @@ EXPORTS = ["c2a_params_default", "c2a_mesh_create", "c2a_mesh_destroy", "c2a_mes
            "c2a_status_string", "c2a_last_error", "c2a_scorer_create", "c2a_scorer_destroy",
            "c2a_scorer_set_params", "c2a_scorer_get_params", "c2a_scorer_infer",
            "c2a_scorer_grad", "c2a_scorer_train", "c2a_scorer_save", "c2a_scorer_load",
-           "c2a_batch_samples", "c2a_ccd_batch_noalloc"]
+           "c2a_batch_samples", "c2a_ccd_batch_noalloc", "c2a_contact_list",
+           "c2a_batch_run_graph"]
 
 
 def lib():
@@ -114,6 +123,7 @@ def lib():
     L.c2a_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Query), i64,
                                    C.POINTER(Params), C.POINTER(vp)]
     L.c2a_batch_run.argtypes = [vp, vp]
+    L.c2a_batch_run_graph.argtypes = [vp, vp]
     L.c2a_batch_fetch.argtypes = [vp, vp, vp]
     L.c2a_batch_destroy.argtypes = [vp]
     L.c2a_batch_last_ms.argtypes = [vp, C.POINTER(C.c_float)]
@@ -123,6 +133,8 @@ def lib():
     L.c2a_ccd_step.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(StepQuery), i64,
                                C.POINTER(StepResult), vp]
     L.c2a_contact_count.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, dp, i64, ip, vp]
+    L.c2a_contact_list.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, dp, i64, C.c_int32, vp,
+                                   ip, vp]
     L.c2a_distance.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, ip, C.c_double, C.c_double,
                                i64, vp, vp]
     L.c2a_collide_distance.argtypes = L.c2a_distance.argtypes
@@ -250,6 +262,10 @@ class Batch:
     def run(self, stream=None):
         _check(lib().c2a_batch_run(self.handle, stream))
 
+    def run_graph(self, stream=None):
+        """The same launch through a captured HIP graph (captured on first use)."""
+        _check(lib().c2a_batch_run_graph(self.handle, stream))
+
     def fetch(self, stream=None):
         out = np.zeros(self.nq, dtype=RESULT_DTYPE)
         _check(lib().c2a_batch_fetch(self.handle, out.ctypes.data_as(C.c_void_p), stream))
@@ -296,6 +312,21 @@ def contact_count(meshes, poses, mesh_pairs, thresholds):
     _check(lib().c2a_contact_count(_mesh_table(meshes), len(meshes), _dp(poses), _ip(mp),
                                    _dp(th), len(th), _ip(out), None))
     return out
+
+
+def contact_list(meshes, poses, mesh_pairs, thresholds, cap):
+    """Contact records (C2A_QueryContact's cont_l, in visiting order): returns
+    (counts (nq,), records (nq, cap) CONTACT_DTYPE); query i's first
+    min(counts[i], cap) records are valid."""
+    poses = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 24)
+    mp = np.ascontiguousarray(mesh_pairs, dtype=np.int32).reshape(-1, 2)
+    th = np.ascontiguousarray(thresholds, dtype=np.float64)
+    counts = np.zeros(len(th), dtype=np.int32)
+    recs = np.zeros((len(th), max(cap, 1)), dtype=CONTACT_DTYPE)
+    _check(lib().c2a_contact_list(_mesh_table(meshes), len(meshes), _dp(poses), _ip(mp), _dp(th),
+                                  len(th), int(cap), recs.ctypes.data_as(C.c_void_p), _ip(counts),
+                                  None))
+    return counts, recs[:, :cap]
 
 
 def distance(meshes, poses, mesh_pairs, seed_tris=None, rel_err=0.0, abs_err=0.0,

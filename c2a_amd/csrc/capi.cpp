@@ -20,6 +20,7 @@
 #include "ccd_core.h"
 #include "ccd_kernel.h"
 #include "scorer.h"
+#include "tuning_table.h"
 
 namespace c2a {
 ScorerDev scorer_view(const c2a_scorer* s);   // scorer.hip
@@ -80,6 +81,11 @@ void unpack_pose(const double* p12, double* R, double* T) {
 
 }  // namespace
 
+namespace c2a {
+// Status with the thread's c2a_last_error text (used by scorer.hip too).
+int set_error(int code, const char* msg) { return fail(code, msg); }
+}  // namespace c2a
+
 struct c2a_mesh {
   Bvh host;
   DevBuf nodes, tris;
@@ -97,7 +103,10 @@ struct c2a_batch {
   int64_t sample_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
+  hipGraphExec_t graph = nullptr;        // captured launch sequence (c2a_batch_run_graph)
+  KArgs tune{};                          // the autotuned bucket knobs (ntune, tune_*)
   ~c2a_batch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
   }
@@ -224,6 +233,36 @@ void params_to_args(KArgs& a, const c2a_params* p) {
   a.translation = (p->translation != 0 && !a.strict) ? 1 : 0;
 }
 
+// The autotuned per-size-bucket knobs (tuning_table.h, generated from
+// c2a_amd/tuning.json by the autotuner): the template depth is the deepest
+// bucket's (5 for the learned order, its one instantiated depth), and the
+// replay arena is sized for the largest log among the buckets the batch's
+// queries use (an explicit replay_log >= 0 overrides every bucket's).
+void apply_tuning(c2a_batch* b, const std::vector<int32_t>& pairs, int replay_log) {
+  if (kTuneRows <= 0) return;
+  const int rows = std::min(kTuneRows, C2A_TUNE_MAX);
+  KArgs& t = b->tune;
+  t.ntune = rows;
+  int kmax = 1;
+  for (int r = 0; r < rows; r++) {
+    t.tune_ntri[r] = kTune[r][0];
+    t.tune_depth[r] = std::min(6, std::max(1, kTune[r][1]));
+    t.tune_miss[r] = std::max(0, kTune[r][2]);
+    t.tune_log[r] = replay_log >= 0 ? replay_log : kTune[r][3];
+    kmax = std::max(kmax, t.tune_depth[r]);
+  }
+  b->K = b->scorer ? 5 : kmax;
+  if (replay_log >= 0) return;
+  int logmax = 0;
+  for (size_t i = 0; i + 1 < pairs.size(); i += 2) {
+    const int big = std::max(b->mtab[(size_t)pairs[i]].ntri, b->mtab[(size_t)pairs[i + 1]].ntri);
+    int r = 0;
+    while (r + 1 < rows && big > t.tune_ntri[r]) r++;
+    logmax = std::max(logmax, t.tune_log[r]);
+  }
+  b->logcap = logmax;
+}
+
 int fill_args(c2a_batch* b, const c2a_params* p) {
   c2a_params def;
   c2a_params_default(&def);
@@ -236,6 +275,13 @@ int fill_args(c2a_batch* b, const c2a_params* p) {
   a.counter = (int64_t*)b->d_counter.p;
   a.nq = b->nq;
   params_to_args(a, p);
+  a.ntune = b->tune.ntune;
+  for (int r = 0; r < C2A_TUNE_MAX; r++) {
+    a.tune_ntri[r] = b->tune.tune_ntri[r];
+    a.tune_depth[r] = b->tune.tune_depth[r];
+    a.tune_miss[r] = b->tune.tune_miss[r];
+    a.tune_log[r] = b->tune.tune_log[r];
+  }
   a.logcap = b->logcap;
   a.arena = (unsigned char*)b->d_arena.p;
   a.feats = (b->scorer || b->sample_cap > 0) ? 1 : 0;
@@ -528,12 +574,30 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     b->logcap = pp->replay_log >= 0 ? pp->replay_log : auto_replay_log(b->mtab);
     b->scorer = pp->order == C2A_ORDER_LEARNED ? pp->scorer : nullptr;
     b->sample_cap = pp->sample_cap;
+    // spec_depth 0: the autotuned knobs of each query's size bucket
+    if (pp->spec_depth == 0) apply_tuning(b, pairs, pp->replay_log);
   }
   b->blocks = grid_blocks(b->K, b->CAP, b->NW, nq, b->scorer != nullptr);
   const bool feats = b->scorer || b->sample_cap > 0;
-  if ((b->logcap > 0 && nq > 0 &&
-       (rc = b->d_arena.alloc((size_t)b->blocks * arena_bytes(b->logcap, feats)))) ||
-      (b->sample_cap > 0 &&
+  if (b->logcap > 0 && nq > 0) {
+    // one replay arena per resident workgroup: fewer workgroups when the
+    // device has no room for all of them
+    const size_t slice = arena_bytes(b->logcap, feats);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) {
+      const size_t fit = (size_t)(0.8 * (double)free_b) / slice;
+      if (fit < (size_t)b->blocks) b->blocks = (int)std::max<size_t>(1, fit);
+    }
+    while ((rc = b->d_arena.alloc((size_t)b->blocks * slice)) && b->blocks > 1) {
+      (void)hipGetLastError();
+      b->blocks /= 2;
+    }
+    if (rc) {
+      delete b;
+      return rc;
+    }
+  }
+  if ((b->sample_cap > 0 &&
        ((rc = b->d_samples.alloc((size_t)b->sample_cap * (NF + 1) * sizeof(float))) ||
         (rc = b->d_keys.alloc((size_t)b->sample_cap * sizeof(int64_t))) ||
         (rc = b->d_scount.alloc(sizeof(int64_t))))) ||
@@ -571,6 +635,46 @@ int c2a_batch_run(c2a_batch* b, void* stream) {
   HIP_TRY(hipEventRecord(b->ev0, s));
   int rc = run_kernel(b->args, b->K, b->CAP, b->NW, b->blocks, s);
   if (rc) return rc;
+  HIP_TRY(hipEventRecord(b->ev1, s));
+  b->timed = true;
+  return C2A_OK;
+}
+
+int c2a_batch_run_graph(c2a_batch* b, void* stream) {
+  if (!b) return fail(C2A_ERR_BAD_ARG, "null batch");
+  if (b->nq == 0) return C2A_OK;
+  if (b->scorer || b->sample_cap > 0)
+    return fail(C2A_ERR_BAD_ARG,
+                "graph replay takes batches without a scorer or sample logging");
+  hipStream_t s = (hipStream_t)stream;
+  if (!b->graph) {
+    // capture the launch sequence (counter reset by a kernel, then the CCD
+    // kernel: no memset node) on a private stream, once per batch
+    hipStream_t cs = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int rc = C2A_OK;
+    if (e == hipSuccess) {
+      rc = run_kernel(b->args, b->K, b->CAP, b->NW, b->blocks, cs);
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(cs, &g);
+      if (!rc && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
+      if (!rc) {
+        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+          b->graph = nullptr;
+          rc = hip_fail(e, "hipGraphInstantiate");
+        }
+      }
+      if (g) (void)hipGraphDestroy(g);
+    } else {
+      rc = hip_fail(e, "hipStreamBeginCapture");
+    }
+    (void)hipStreamDestroy(cs);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventRecord(b->ev0, s));
+  HIP_TRY(hipGraphLaunch(b->graph, s));
   HIP_TRY(hipEventRecord(b->ev1, s));
   b->timed = true;
   return C2A_OK;
@@ -711,11 +815,12 @@ int c2a_ccd_step(c2a_mesh* const* meshes, int32_t nmesh, const c2a_step_query* q
   return C2A_OK;
 }
 
-int c2a_contact_count(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
-                      const int32_t* mesh_pairs, const double* thresholds, int64_t nq,
-                      int32_t* counts_out, void* stream) {
+static int contact_queries(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                           const int32_t* mesh_pairs, const double* thresholds, int64_t nq,
+                           int32_t cap, c2a_contact* out, int32_t* counts_out, void* stream) {
   if (nq < 0 || (nq > 0 && (!poses || !mesh_pairs || !thresholds || !counts_out)))
     return fail(C2A_ERR_BAD_ARG, "bad arrays");
+  if (cap < 0 || (cap > 0 && !out)) return fail(C2A_ERR_BAD_ARG, "bad contact buffer");
   if (nq == 0) return C2A_OK;
   int rc = check_device();
   if (rc) return rc;
@@ -737,10 +842,11 @@ int c2a_contact_count(c2a_mesh* const* meshes, int32_t nmesh, const double* pose
   int CAP = pick_cap(tab, mesh_pairs, nq, 2);
   if (CAP < 0) return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
   const int K = 5;
-  DevBuf dm, din, dout, dc;
+  DevBuf dm, din, dout, dc, drec;
   if ((rc = dm.alloc(tab.size() * sizeof(MeshDev))) ||
       (rc = din.alloc((size_t)nq * sizeof(ContactDev))) ||
-      (rc = dout.alloc((size_t)nq * sizeof(int32_t))) || (rc = dc.alloc(sizeof(int64_t))))
+      (rc = dout.alloc((size_t)nq * sizeof(int32_t))) || (rc = dc.alloc(sizeof(int64_t))) ||
+      (cap > 0 && (rc = drec.alloc((size_t)nq * cap * sizeof(c2a_contact)))))
     return rc;
   hipStream_t s = (hipStream_t)stream;
   HIP_TRY(hipMemcpy(dm.p, tab.data(), tab.size() * sizeof(MeshDev), hipMemcpyHostToDevice));
@@ -750,13 +856,53 @@ int c2a_contact_count(c2a_mesh* const* meshes, int32_t nmesh, const double* pose
   a.meshes = (const MeshDev*)dm.p;
   a.contact_in = (const ContactDev*)din.p;
   a.counts_out = (int32_t*)dout.p;
+  a.contacts = cap > 0 ? (c2a_contact*)drec.p : nullptr;
+  a.contact_cap = cap;
   a.counter = (int64_t*)dc.p;
   a.nq = nq;
   a.mode = MODE_CONTACT;
   if ((rc = run_kernel(a, K, CAP, 1, grid_blocks(K, CAP, 1, nq), s))) return rc;
   HIP_TRY(hipMemcpyAsync(counts_out, dout.p, (size_t)nq * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (cap > 0)
+    HIP_TRY(hipMemcpyAsync(out, drec.p, (size_t)nq * cap * sizeof(c2a_contact),
+                           hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  // feature slots -> the caller's vertex ids (AddTri i1..i3, build order),
+  // as C2A.cpp:1565-1604 reads them from the triangles' index triples
+  for (int64_t i = 0; cap > 0 && i < nq; i++) {
+    const int n = std::min<int>(counts_out[i], cap);
+    for (int k = 0; k < n; k++) {
+      c2a_contact& c = out[i * (int64_t)cap + k];
+      for (int side = 0; side < 2; side++) {
+        const Bvh& h = meshes[mesh_pairs[2 * i + side]]->host;
+        const int32_t* v = &h.tri_vidx[3 * (size_t)c.tri[side]];
+        const int f = c.fid[side][0];
+        for (int j = 0; j < 3; j++) c.fid[side][j] = -1;
+        if (c.type[side] == 1) c.fid[side][0] = v[f];
+        else if (c.type[side] == 2) {
+          c.fid[side][0] = v[f];
+          c.fid[side][1] = v[(f + 1) % 3];
+        } else if (c.type[side] == 3) {
+          for (int j = 0; j < 3; j++) c.fid[side][j] = v[j];
+        }
+      }
+    }
+  }
   return C2A_OK;
+}
+
+int c2a_contact_count(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                      const int32_t* mesh_pairs, const double* thresholds, int64_t nq,
+                      int32_t* counts_out, void* stream) {
+  return contact_queries(meshes, nmesh, poses, mesh_pairs, thresholds, nq, 0, nullptr,
+                         counts_out, stream);
+}
+
+int c2a_contact_list(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                     const int32_t* mesh_pairs, const double* thresholds, int64_t nq,
+                     int32_t cap, c2a_contact* out, int32_t* counts_out, void* stream) {
+  return contact_queries(meshes, nmesh, poses, mesh_pairs, thresholds, nq, cap, out, counts_out,
+                         stream);
 }
 
 static int distance_queries(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,

@@ -282,6 +282,8 @@ struct Leader {
   int spos, nev, logfull;        // scan position, events, log capacity reached
   int napp;                      // next free record pair of the arena
   int need_score;                // replayed records wait for the scorer
+  int kdepth, kmiss, klog;       // this query's speculative depth, depth below
+                                 // a miss and replay log capacity
   unsigned char* ab;             // this wavefront's replay arena
 };
 
@@ -664,7 +666,7 @@ C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, 
       // a replayed pair that must now be expanded (a miss) usually needs
       // only a level or two: a shallow first round has a shorter chain of
       // dependent node loads
-      g.rdepth = ((flags & 8) && a.miss_depth > 0 && a.miss_depth < K) ? a.miss_depth : K;
+      g.rdepth = ((flags & 8) && L.kmiss > 0 && L.kmiss < L.kdepth) ? L.kmiss : L.kdepth;
     }
   }
   if (to_arena) {
@@ -995,7 +997,7 @@ C2A_HD void replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl)
     for (int pass = 0; pass < a.ext_levels; pass++) {
       if (lane == 0) g.next = 0;
       lane_sync();
-      const int room = a.logcap - ntot;
+      const int room = L.klog - ntot;
       for (int r = lo + lane; r < hi; r += nl) {
         const HotRec h = hot[r];
         if ((h.flags & 3) == 3 || cpa[r] >= 0) continue;
@@ -1230,7 +1232,7 @@ C2A_HD int scan(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
     L.bv += 2 * nexp;
     L.tri += (int)__builtin_popcountll(wave_ballot(in && tested));
     // log the expansions in depth-first order (next step's replay)
-    if (!L.logfull && L.nlog + nexp > a.logcap) L.logfull = 1;
+    if (!L.logfull && L.nlog + nexp > L.klog) L.logfull = 1;
     const int en = (expand && !L.logfull) ? L.nlog + lanes_below(em, j) : -1;
     const int pl = E.ppos - t0;
     const int pen_in = lane_bcast(en, pl >= 0 ? pl : 0);
@@ -1328,6 +1330,36 @@ C2A_HD void flush_log(Wave<K, CAP>& g, Leader& L, const ArenaView& av, int lane,
   for (int i = lane; i < n; i += nl) lw[i] = g.lbuf[i];
   L.nflush = L.nlog;
   lane_sync();
+}
+
+// One contact record (C2A.cpp:1561-1631) of triangle pair (t1, t2) at the
+// contact traversal's pose: the features and closest points of the same
+// triangle distance the leaf test made.  Feature ids are the vertex slots
+// 0..2 of the triangle; the host maps them to the caller's vertex ids.
+template <int K, int CAP>
+C2A_HD void record_contact(const Wave<K, CAP>& g, c2a_contact* out, int t1, int t2) {
+  const MeshDev M1 = mesh_of(g, 0);
+  const MeshDev M2 = mesh_of(g, 1);
+  double ta[9], tb[9], Rrel[9], Trel[3], p[3], q[3], u[3], pb[3];
+  gdouble_ptr g1 = tris_of(M1) + 9 * (int64_t)t1;
+  gdouble_ptr g2 = tris_of(M2) + 9 * (int64_t)t2;
+  for (int i = 0; i < 9; i++) { ta[i] = g1[i]; tb[i] = g2[i]; Rrel[i] = g.Rrel[i]; }
+  for (int i = 0; i < 3; i++) Trel[i] = g.Trel[i];
+  int feat[4];
+  const double d = tri_distance(Rrel, Trel, ta, tb, p, q, feat);
+  sub3(u, q, Trel);
+  mtxv(pb, Rrel, u);
+  c2a_contact c;
+  c.tri[0] = t1;
+  c.tri[1] = t2;
+  c.type[0] = feat[0] + 1;
+  c.type[1] = feat[2] + 1;
+  for (int k = 0; k < 3; k++) { c.fid[0][k] = -1; c.fid[1][k] = -1; }
+  c.fid[0][0] = feat[1];
+  c.fid[1][0] = feat[3];
+  for (int i = 0; i < 3; i++) { c.pa[i] = p[i]; c.pb[i] = pb[i]; }
+  c.distance = d;
+  *gp(out) = c;
 }
 
 // Puts the pair being processed back on the stack (a round will be rooted
@@ -1437,7 +1469,12 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
         }
         L.tri++;
       } else {
-        if ((uni(sl.flags) & 4) && unid(sl.dTri) <= L.thr) L.num_contact++;
+        if ((uni(sl.flags) & 4) && unid(sl.dTri) <= L.thr) {
+          if (a.contacts && L.num_contact < a.contact_cap && lane == 0)
+            record_contact(g, a.contacts + L.qi * (int64_t)a.contact_cap + L.num_contact,
+                           uni(sl.t1), uni(sl.t2));
+          L.num_contact++;
+        }
       }
       continue;
     }
@@ -1457,7 +1494,7 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
     if (!contact || L.mode == MODE_DISTANCE) L.bv += 2;
     // log the expansion for the next step's replay
     int nnA = -1;
-    if (c.nn >= 0 && L.nlog < a.logcap) {
+    if (c.nn >= 0 && L.nlog < L.klog) {
       const int en = L.nlog++;
       if (lane == 0) {
         LogEnt v;
@@ -1617,6 +1654,11 @@ C2A_HD void finish_query(Wave<K, CAP>& g, Leader& L, const KArgs& a, bool collis
     for (int i = 0; i < 9; i++) r->R_toc[i] = collisionfree ? 0.0 : Rt[i];
     for (int i = 0; i < 3; i++) r->T_toc[i] = collisionfree ? 0.0 : Tt[i];
     r->num_contact = 0;
+    for (int i = 0; i < 9; i++) r->R[i] = g.Rrel[i];
+    for (int i = 0; i < 3; i++) r->T[i] = g.Trel[i];
+    r->abs_err = L.abs_err;
+    r->rel_err = L.rel_err;
+    r->UpboundTOC = L.Up;
   }
   if (!collisionfree && a.with_contacts) {
     const MeshDev M1 = mesh_of(g, 0);
@@ -1652,6 +1694,8 @@ C2A_HD void after_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
       r->last_tri[0] = L.last1;
       r->last_tri[1] = L.last2;
       r->flags = L.flags;
+      for (int i = 0; i < 9; i++) r->R[i] = g.Rrel[i];
+      for (int i = 0; i < 3; i++) r->T[i] = g.Trel[i];
     }
     L.state = ST_FETCH;
     return;
@@ -1720,6 +1764,9 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     return false;
   }
   L.qi = qi;
+  L.kdepth = K;
+  L.kmiss = a.miss_depth;
+  L.klog = a.logcap;
   L.flags = 0;
   L.bv = 0;
   L.tri = 0;
@@ -1860,6 +1907,15 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   L.seed2 = s2;
   L.last1 = s1;   // model last_tri starts at the seed (demo protocol)
   L.last2 = s2;
+  if (a.ntune > 0) {   // the size bucket of the larger mesh
+    const int n1 = uni(g.md[0].ntri), n2 = uni(g.md[1].ntri);
+    const int big = n1 > n2 ? n1 : n2;
+    int b = 0;
+    while (b + 1 < a.ntune && big > a.tune_ntri[b]) b++;
+    L.kdepth = a.tune_depth[b] < K ? a.tune_depth[b] : K;
+    L.kmiss = a.tune_miss[b];
+    L.klog = a.tune_log[b] < a.logcap ? a.tune_log[b] : a.logcap;
+  }
   L.mode = MODE_QUERY;
   L.nlog_prev = 0;
   L.logbuf = 0;
@@ -1999,6 +2055,17 @@ C2A_HD void translation_query(Wave<K, CAP>& g, Leader& L, const KArgs& a, int la
     load_tri(tb, M2, lb);
     L.D = tri_distance(Rrel, Trel, ta, tb, L.p1, L.p2, nullptr);
   }
+  lane_sync();
+  if (lane == 0) {   // res->R, res->T at the advanced pose (C2A.cpp:1902-1904)
+    for (int i = 0; i < 9; i++) g.Rrel[i] = Rrel[i];
+    for (int i = 0; i < 3; i++) g.Trel[i] = Trel[i];
+  }
+  lane_sync();
+  // this branch leaves the models' last_tri alone and updates the result's
+  // last_triA/B to the last improving pair (C2A.cpp:1413-1414): the result
+  // reports that pair
+  L.last1 = la;
+  L.last2 = lb;
   L.toc = L.M;
   L.numCA = 0;
   finish_query(g, L, a, L.toc >= 1.0, lane);

@@ -14,6 +14,8 @@ namespace c2a {
 
 enum : int { MODE_QUERY = 0, MODE_STEP = 1, MODE_CONTACT = 2, MODE_DISTANCE = 3 };
 
+constexpr int C2A_TUNE_MAX = 8;   // size buckets of the autotuned knobs
+
 struct MeshDev {
   const GpuNode* nodes;
   const double* tris;      // build order, 9 doubles each
@@ -172,6 +174,9 @@ struct KArgs {
   c2a_step_result* step_out;
   const ContactDev* contact_in;
   int32_t* counts_out;
+  c2a_contact* contacts;   // contact records, contact_cap per query (or null)
+  int32_t contact_cap;
+  int32_t pad1;
   c2a_distance_result* dist_out;
   int64_t* counter;        // work-queue ticket, zeroed before each launch
   int64_t nq;
@@ -191,6 +196,15 @@ struct KArgs {
                            // whose two rotations are below 1e-8
   int32_t strict;          // strict (conservative) mode: every tested leaf
                            // pair bounds mint (motions carry the world axis)
+  // Per-size-bucket traversal knobs (autotuned): a CA query takes the first
+  // bucket whose triangle bound covers its larger mesh and uses that
+  // bucket's speculative depth (<= K), depth below a miss and replay log
+  // (<= logcap).  ntune == 0: every query uses K, miss_depth and logcap.
+  int32_t ntune;
+  int32_t tune_ntri[C2A_TUNE_MAX];
+  int32_t tune_depth[C2A_TUNE_MAX];
+  int32_t tune_miss[C2A_TUNE_MAX];
+  int32_t tune_log[C2A_TUNE_MAX];
   ScorerDev scorer;        // learned child order when scorer.on
   // training samples of replayed pairs: a pair is logged when the top 32
   // bits of mix64(key) are below sample_thr (2^32 = every pair)

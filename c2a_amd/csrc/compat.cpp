@@ -71,6 +71,53 @@ void copy_result(const c2a_result& r, C2A_TimeOfContactResult* res) {
   res->numCA = r.numCA;
   res->collisionfree = r.collisionfree != 0;
   res->num_contact = r.num_contact;
+  for (int i = 0; i < 3; i++) {
+    res->T[i] = r.T[i];
+    for (int j = 0; j < 3; j++) res->R[i][j] = r.R[3 * i + j];
+  }
+  res->abs_err = r.abs_err;
+  res->rel_err = r.rel_err;
+  res->UpboundTOC = r.UpboundTOC;
+}
+
+// C2A_QueryContact at absolute poses (C2A.cpp:1937-1966 ->
+// C2A_TimeOfContactStep_Contact, 1727-1775): res->R/T at these poses, the
+// error schedule zeroed, mint 1, UpboundTOC 1, and the contact pairs within
+// `threshold` in cont_l (the reference's push_front order).  Returns the
+// engine status.
+int contact_query(c2a_mesh* const* meshes, const double* poses24, double threshold,
+                  C2A_TimeOfContactResult* res) {
+  const int32_t pairs[2] = {0, 1};
+  int32_t count = 0;
+  res->num_contact = 0;
+  res->UpboundTOC = 1;
+  res->cont_l.clear();
+  if (c2a_contact_count(meshes, 2, poses24, pairs, &threshold, 1, &count, nullptr) != C2A_OK)
+    return -1;
+  std::vector<c2a_contact> recs((size_t)count);
+  if (count > 0 && c2a_contact_list(meshes, 2, poses24, pairs, &threshold, 1, count, recs.data(),
+                                    &count, nullptr) != C2A_OK)
+    return -1;
+  for (int k = 0; k < count; k++) {
+    c2a_contact& c = recs[(size_t)k];
+    res->cont_l.push_front(ContactF(c.type[0], c.type[1], c.fid[0], c.fid[1], c.tri[0], c.tri[1],
+                                    c.pa, c.pb, c.distance));
+  }
+  res->num_contact = count;
+  double R1[9], T1[3], R2[9], T2[3], Rr[9], Tr[3], u[3];
+  for (int i = 0; i < 9; i++) { R1[i] = poses24[i]; R2[i] = poses24[12 + i]; }
+  for (int i = 0; i < 3; i++) { T1[i] = poses24[9 + i]; T2[i] = poses24[21 + i]; }
+  c2a::mtxm(Rr, R1, R2);
+  c2a::sub3(u, T2, T1);
+  c2a::mtxv(Tr, R1, u);
+  for (int i = 0; i < 3; i++) {
+    res->T[i] = Tr[i];
+    for (int j = 0; j < 3; j++) res->R[i][j] = Rr[3 * i + j];
+  }
+  res->rel_err = 0;
+  res->abs_err = 0;
+  res->mint = 1;
+  return C2A_OK;
 }
 
 }  // namespace
@@ -235,7 +282,10 @@ PQP_REAL C2A_QueryTimeOfContact(CInterpMotion* m1, CInterpMotion* m2, C2A_TimeOf
   // after a translation-only C2A_Solve; that leftover state is not kept.)
   p.translation = 0;
   c2a_result r;
+  const auto t0 = std::chrono::steady_clock::now();
   if (c2a_ccd_batch(meshes, 2, &q, 1, &p, &r, nullptr) != C2A_OK) return -1.0;
+  res->query_time_secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   copy_result(r, res);
   res->num_contact = 0;
   set_last(o1, r.last_tri[0]);
@@ -271,23 +321,41 @@ C2A_Result C2A_Solve(Transform* trans00, Transform* trans01, C2A_Model* obj1, Tr
   c2a_mesh* meshes[2] = {obj1->engine_mesh(), obj2->engine_mesh()};
   c2a_query q = make_query(&motion1, &motion2, &dres, obj1, obj2);
   c2a_params p;
-  c2a_params_default(&p);   // 1e-4 / 1e-4, contacts within 2 d + 0.001
+  c2a_params_default(&p);   // 1e-4 / 1e-4
+  p.with_contacts = 0;      // the contact query below counts and lists them
   c2a_result r;
   number_of_iteration = 0;
+  const auto t0 = std::chrono::steady_clock::now();
   if (c2a_ccd_batch(meshes, 2, &q, 1, &p, &r, nullptr) != C2A_OK) return CollisionNotFound;
   copy_result(r, &dres);
   dres.cont_l.clear();
-  set_last(obj1, r.last_tri[0]);
-  set_last(obj2, r.last_tri[1]);
+  if (r.flags & C2A_FLAG_TRANSLATION_ONLY) {
+    // the translation-only branch updates dres.last_triA/B to its last
+    // improving pair and leaves the models' last_tri (C2A.cpp:1413-1414)
+    if (r.last_tri[0] >= 0 && r.last_tri[0] < obj1->num_tris) dres.last_triA = obj1->tris + r.last_tri[0];
+    if (r.last_tri[1] >= 0 && r.last_tri[1] < obj2->num_tris) dres.last_triB = obj2->tris + r.last_tri[1];
+  } else {
+    set_last(obj1, r.last_tri[0]);
+    set_last(obj2, r.last_tri[1]);
+  }
+  dres.num_contact = 0;
   if (!dres.collisionfree) {
-    double pb[12];
+    double pb[12], poses[24];
     for (int i = 0; i < 3; i++) {
       trans0.Translation()[i] = r.T_toc[i];
       for (int j = 0; j < 3; j++) trans0.Rotation()[i][j] = r.R_toc[3 * i + j];
     }
     c2a_motion_pose(motion2.engine_motion(), dres.toc, pb, nullptr);
     transform_from(pb, trans1);
+    // C2A_QueryContact at the TOC pose within 2 d + 0.001 (C2A.cpp:2433-2435)
+    for (int i = 0; i < 9; i++) poses[i] = r.R_toc[i];
+    for (int i = 0; i < 3; i++) poses[9 + i] = r.T_toc[i];
+    for (int i = 0; i < 12; i++) poses[12 + i] = pb[i];
+    if (contact_query(meshes, poses, 2 * dres.distance + 0.001, &dres) != C2A_OK)
+      return CollisionNotFound;
   }
+  dres.query_time_secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   time_of_contact = dres.toc;
   number_of_contact = dres.num_contact;
   number_of_iteration = dres.numCA;
@@ -332,6 +400,8 @@ int C2A_TimeOfContactStep(CInterpMotion* m1, CInterpMotion* m2, C2A_TimeOfContac
   for (int i = 0; i < 3; i++) {
     res->p1[i] = r.p1[i];
     res->p2[i] = r.p2[i];
+    res->T[i] = r.T[i];
+    for (int j = 0; j < 3; j++) res->R[i][j] = r.R[3 * i + j];
   }
   res->num_bv_tests += (int)r.bv_tests;
   res->num_tri_tests += (int)r.tri_tests;
@@ -406,12 +476,6 @@ PQP_REAL C2A_QueryContact(CInterpMotion* m1, CInterpMotion* m2, C2A_TimeOfContac
   double poses[24];
   pose12_from(m1->transform, poses);
   pose12_from(m2->transform, poses + 12);
-  const int32_t pairs[2] = {0, 1};
-  int32_t count = 0;
-  res->num_contact = 0;
-  res->UpboundTOC = 1;
-  if (c2a_contact_count(meshes, 2, poses, pairs, &threshold, 1, &count, nullptr) != C2A_OK)
-    return -1.0;
-  res->num_contact = count;
+  if (contact_query(meshes, poses, threshold, res) != C2A_OK) return -1.0;
   return PQP_OK;
 }

@@ -168,9 +168,27 @@ struct c2a_scorer {
   }
 };
 
+namespace c2a {
+int set_error(int code, const char* msg);   // capi.cpp: sets c2a_last_error
+}
+
 namespace {
 
-int hip_err(hipError_t e) { return e == hipSuccess ? C2A_OK : C2A_ERR_HIP; }
+int hip_err(hipError_t e) {
+  return e == hipSuccess ? C2A_OK : c2a::set_error(C2A_ERR_HIP, hipGetErrorString(e));
+}
+int bad_arg(const char* what) { return c2a::set_error(C2A_ERR_BAD_ARG, what); }
+int oom(const char* what) { return c2a::set_error(C2A_ERR_OOM, what); }
+
+// Device scratch of one call, freed on every return path.
+template <class T>
+struct DevArray {
+  T* p = nullptr;
+  ~DevArray() {
+    if (p) (void)hipFree(p);
+  }
+  bool alloc(size_t n) { return hipMalloc(&p, (n ? n : 1) * sizeof(T)) == hipSuccess; }
+};
 
 ScorerDev dev_view(const c2a_scorer* s, float b3) {
   ScorerDev d;
@@ -211,12 +229,13 @@ ScorerDev scorer_view(const c2a_scorer* s) { return dev_view(s, read_b3(s)); }
 extern "C" {
 
 int c2a_scorer_create(uint64_t seed, c2a_scorer** out) {
-  if (!out) return C2A_ERR_BAD_ARG;
+  if (!out) return bad_arg("null output handle");
   *out = nullptr;
   int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return C2A_ERR_NO_DEVICE;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return c2a::set_error(C2A_ERR_NO_DEVICE, "no HIP device available");
   c2a_scorer* s = new (std::nothrow) c2a_scorer();
-  if (!s) return C2A_ERR_OOM;
+  if (!s) return oom("host allocation failed");
   const size_t pb = (NPARAM + 1) * sizeof(float);
   if (hipMalloc(&s->p, pb) != hipSuccess || hipMalloc(&s->mom, pb) != hipSuccess ||
       hipMalloc(&s->vel, pb) != hipSuccess ||
@@ -225,7 +244,7 @@ int c2a_scorer_create(uint64_t seed, c2a_scorer** out) {
       hipMalloc(&s->w1t, NF * NH * sizeof(uint16_t)) != hipSuccess ||
       hipMalloc(&s->w2t, NH * NH * sizeof(uint16_t)) != hipSuccess) {
     delete s;
-    return C2A_ERR_OOM;
+    return oom("scorer parameter buffers");
   }
   // He-uniform initial weights, zero biases
   std::vector<float> h(NPARAM + 1, 0.0f);
@@ -252,9 +271,9 @@ int c2a_scorer_destroy(c2a_scorer* s) {
 }
 
 int c2a_scorer_set_params(c2a_scorer* s, const float* params) {
-  if (!s || !params) return C2A_ERR_BAD_ARG;
-  if (hipMemcpy(s->p, params, NPARAM * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
-    return C2A_ERR_HIP;
+  if (!s || !params) return bad_arg("null scorer or parameters");
+  const hipError_t e0 = hipMemcpy(s->p, params, NPARAM * sizeof(float), hipMemcpyHostToDevice);
+  if (e0 != hipSuccess) return hip_err(e0);
   (void)hipMemset(s->mom, 0, (NPARAM + 1) * sizeof(float));
   (void)hipMemset(s->vel, 0, (NPARAM + 1) * sizeof(float));
   s->step = 0;
@@ -264,19 +283,19 @@ int c2a_scorer_set_params(c2a_scorer* s, const float* params) {
 }
 
 int c2a_scorer_get_params(const c2a_scorer* s, float* params) {
-  if (!s || !params) return C2A_ERR_BAD_ARG;
+  if (!s || !params) return bad_arg("null scorer or parameters");
   return hip_err(hipMemcpy(params, s->p, NPARAM * sizeof(float), hipMemcpyDeviceToHost));
 }
 
 // File format: the 8 bytes "C2ASCR01", the parameter count as a
 // little-endian uint32, then the FP32 parameters (little-endian).
 int c2a_scorer_save(const c2a_scorer* s, const char* path) {
-  if (!s || !path) return C2A_ERR_BAD_ARG;
+  if (!s || !path) return bad_arg("null scorer or path");
   std::vector<float> h(NPARAM);
   int rc = c2a_scorer_get_params(s, h.data());
   if (rc) return rc;
   FILE* f = fopen(path, "wb");
-  if (!f) return C2A_ERR_BAD_ARG;
+  if (!f) return bad_arg("cannot open the scorer file for writing");
   const uint32_t n = NPARAM;
   const bool ok = fwrite("C2ASCR01", 1, 8, f) == 8 && fwrite(&n, 4, 1, f) == 1 &&
                   fwrite(h.data(), sizeof(float), NPARAM, f) == (size_t)NPARAM;
@@ -284,10 +303,10 @@ int c2a_scorer_save(const c2a_scorer* s, const char* path) {
 }
 
 int c2a_scorer_load(const char* path, c2a_scorer** out) {
-  if (!path || !out) return C2A_ERR_BAD_ARG;
+  if (!path || !out) return bad_arg("null path or output handle");
   *out = nullptr;
   FILE* f = fopen(path, "rb");
-  if (!f) return C2A_ERR_BAD_ARG;
+  if (!f) return bad_arg("cannot open the scorer file");
   char magic[8];
   uint32_t n = 0;
   std::vector<float> h(NPARAM);
@@ -295,9 +314,9 @@ int c2a_scorer_load(const char* path, c2a_scorer** out) {
                   fread(&n, 4, 1, f) == 1 && n == (uint32_t)NPARAM &&
                   fread(h.data(), sizeof(float), NPARAM, f) == (size_t)NPARAM;
   fclose(f);
-  if (!ok) return C2A_ERR_BAD_ARG;
+  if (!ok) return bad_arg("not a scorer file (header, count or size)");
   for (float v : h)
-    if (!std::isfinite(v)) return C2A_ERR_BAD_ARG;
+    if (!std::isfinite(v)) return bad_arg("scorer file holds a non-finite parameter");
   c2a_scorer* s = nullptr;
   int rc = c2a_scorer_create(0, &s);
   if (rc) return rc;
@@ -311,24 +330,18 @@ int c2a_scorer_load(const char* path, c2a_scorer** out) {
 }
 
 int c2a_scorer_infer(const c2a_scorer* s, const float* feats, int64_t n, float* out) {
-  if (!s || n < 0 || (n > 0 && (!feats || !out))) return C2A_ERR_BAD_ARG;
+  if (!s || n < 0 || (n > 0 && (!feats || !out))) return bad_arg("bad scorer arguments");
   if (n == 0) return C2A_OK;
-  float *dx = nullptr, *dy = nullptr;
-  if (hipMalloc(&dx, n * NF * sizeof(float)) != hipSuccess) return C2A_ERR_OOM;
-  if (hipMalloc(&dy, n * sizeof(float)) != hipSuccess) {
-    (void)hipFree(dx);
-    return C2A_ERR_OOM;
-  }
-  hipError_t e = hipMemcpy(dx, feats, n * NF * sizeof(float), hipMemcpyHostToDevice);
+  DevArray<float> dx, dy;
+  if (!dx.alloc((size_t)n * NF) || !dy.alloc((size_t)n)) return oom("scorer device buffers");
+  hipError_t e = hipMemcpy(dx.p, feats, n * NF * sizeof(float), hipMemcpyHostToDevice);
   const int grid = (int)((n + 63) / 64 < 4096 ? (n + 63) / 64 : 4096);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(infer_kernel, dim3(grid), dim3(64), 0, 0, dev_view(s, read_b3(s)), dx, n,
-                       dy);
+    hipLaunchKernelGGL(infer_kernel, dim3(grid), dim3(64), 0, 0, dev_view(s, read_b3(s)), dx.p,
+                       n, dy.p);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpy(out, dy, n * sizeof(float), hipMemcpyDeviceToHost);
-  (void)hipFree(dx);
-  (void)hipFree(dy);
+  if (e == hipSuccess) e = hipMemcpy(out, dy.p, n * sizeof(float), hipMemcpyDeviceToHost);
   return hip_err(e);
 }
 
@@ -336,49 +349,46 @@ int c2a_scorer_infer(const c2a_scorer* s, const float* feats, int64_t n, float* 
 // mean squared error itself.
 int c2a_scorer_grad(const c2a_scorer* s, const float* feats, const float* labels, int64_t n,
                     float* grad_out, float* loss_out) {
-  if (!s || n <= 0 || !feats || !labels || !grad_out) return C2A_ERR_BAD_ARG;
+  if (!s || n <= 0 || !feats || !labels || !grad_out) return bad_arg("bad scorer arguments");
   const int per_block = 256;
   const int nb = (int)((n + per_block - 1) / per_block);
-  float *dx = nullptr, *dy = nullptr, *part = nullptr, *g = nullptr;
-  if (hipMalloc(&dx, n * NF * sizeof(float)) != hipSuccess ||
-      hipMalloc(&dy, n * sizeof(float)) != hipSuccess ||
-      hipMalloc(&part, (size_t)nb * (NPARAM + 1) * sizeof(float)) != hipSuccess ||
-      hipMalloc(&g, (NPARAM + 1) * sizeof(float)) != hipSuccess)
-    return C2A_ERR_OOM;
-  hipError_t e = hipMemcpy(dx, feats, n * NF * sizeof(float), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(dy, labels, n * sizeof(float), hipMemcpyHostToDevice);
+  DevArray<float> dx, dy, part, g;
+  if (!dx.alloc((size_t)n * NF) || !dy.alloc((size_t)n) ||
+      !part.alloc((size_t)nb * (NPARAM + 1)) || !g.alloc(NPARAM + 1))
+    return oom("scorer gradient buffers");
+  hipError_t e = hipMemcpy(dx.p, feats, n * NF * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dy.p, labels, n * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(grad_kernel, dim3(nb), dim3(64), 0, 0, dev_view(s, read_b3(s)), dx, dy,
-                       (const int32_t*)nullptr, n, per_block, (float)(2.0 / (double)n), part);
-    hipLaunchKernelGGL(reduce_kernel, dim3((NPARAM + 256) / 256), dim3(256), 0, 0, part, nb, g);
+    hipLaunchKernelGGL(grad_kernel, dim3(nb), dim3(64), 0, 0, dev_view(s, read_b3(s)), dx.p,
+                       dy.p, (const int32_t*)nullptr, n, per_block, (float)(2.0 / (double)n),
+                       part.p);
+    hipLaunchKernelGGL(reduce_kernel, dim3((NPARAM + 256) / 256), dim3(256), 0, 0, part.p, nb,
+                       g.p);
     e = hipGetLastError();
   }
   std::vector<float> h(NPARAM + 1);
-  if (e == hipSuccess) e = hipMemcpy(h.data(), g, (NPARAM + 1) * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    e = hipMemcpy(h.data(), g.p, (NPARAM + 1) * sizeof(float), hipMemcpyDeviceToHost);
   if (e == hipSuccess) {
     memcpy(grad_out, h.data(), NPARAM * sizeof(float));
     if (loss_out) *loss_out = (float)(h[NPARAM] / (double)n);
   }
-  (void)hipFree(dx);
-  (void)hipFree(dy);
-  (void)hipFree(part);
-  (void)hipFree(g);
   return hip_err(e);
 }
 
 int c2a_scorer_train(c2a_scorer* s, const float* feats, const float* labels, int64_t n,
                      int32_t epochs, int32_t batch, float lr, uint64_t seed, float* loss_out) {
-  if (!s || n <= 0 || !feats || !labels || epochs <= 0 || batch <= 0) return C2A_ERR_BAD_ARG;
+  if (!s || n <= 0 || !feats || !labels || epochs <= 0 || batch <= 0)
+    return bad_arg("bad scorer training arguments");
   const int per_block = 256;
   const int nbmax = (batch + per_block - 1) / per_block;
-  float *dx = nullptr, *dy = nullptr, *part = nullptr, *g = nullptr;
-  int32_t* didx = nullptr;
-  if (hipMalloc(&dx, n * NF * sizeof(float)) != hipSuccess ||
-      hipMalloc(&dy, n * sizeof(float)) != hipSuccess ||
-      hipMalloc(&didx, n * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&part, (size_t)nbmax * (NPARAM + 1) * sizeof(float)) != hipSuccess ||
-      hipMalloc(&g, (NPARAM + 1) * sizeof(float)) != hipSuccess)
-    return C2A_ERR_OOM;
+  DevArray<float> dx_, dy_, part_, g_;
+  DevArray<int32_t> didx_;
+  if (!dx_.alloc((size_t)n * NF) || !dy_.alloc((size_t)n) || !didx_.alloc((size_t)n) ||
+      !part_.alloc((size_t)nbmax * (NPARAM + 1)) || !g_.alloc(NPARAM + 1))
+    return oom("scorer training buffers");
+  float *dx = dx_.p, *dy = dy_.p, *part = part_.p, *g = g_.p;
+  int32_t* didx = didx_.p;
   hipError_t e = hipMemcpy(dx, feats, n * NF * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dy, labels, n * sizeof(float), hipMemcpyHostToDevice);
   std::vector<int32_t> perm((size_t)n);
@@ -411,11 +421,6 @@ int c2a_scorer_train(c2a_scorer* s, const float* feats, const float* labels, int
     }
     if (loss_out) loss_out[ep] = (float)(eloss / (double)n);
   }
-  (void)hipFree(dx);
-  (void)hipFree(dy);
-  (void)hipFree(didx);
-  (void)hipFree(part);
-  (void)hipFree(g);
   return hip_err(e);
 }
 

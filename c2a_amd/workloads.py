@@ -249,8 +249,37 @@ def knot_queries(nq, seed, collide_frac=0.0):
 
 CONFIG_KNOT_SOLVE = dict(name="knot-pair sequential C2A_Solve parity", nq=1000, seed=20260,
                          tol_d=1e-3, tol_t=1e-4, collide_frac=0.0)
+# The knot batch's queries: knot_batch_queries() (about 30% colliding).  The
+# scorer's training data used knot_queries(4096, seed, 0.3) with this seed.
 CONFIG_KNOT_BATCH = dict(name="knot-pair batched CCD", nq=4096, seed=20261,
-                         tol_d=1e-3, tol_t=1e-4, collide_frac=0.3)
+                         tol_d=1e-3, tol_t=1e-4, collide_frac=0.3,
+                         fixture="knot_batch_4096.npz")
+
+
+def knot_batch_queries(nq=None, path=None):
+    """The knot batch (4096 queries, about 30% colliding), or its first nq.
+
+    Poses as knot_queries: translation U[-2,2]^3 and uniform rotations, with
+    the colliding queries' end pose of object 2 pulled toward object 1's by
+    the seeded lerp.  Two radius-1 knots that close collide on most plain
+    draws, so the batch takes, per query, either the next colliding pulled
+    candidate or the next collision-free plain candidate of two seeded
+    streams; which candidates those are was classified once with the FP64
+    oracle and is stored in tests/golden/knot_batch_4096.npz (written by
+    tests/golden/make_knot_batch.py).  Returns (P00, P01, P10, P11)."""
+    d = np.load(path or os.path.join(_FIXTURES, CONFIG_KNOT_BATCH["fixture"]))
+    seed_free, seed_hit, _ = (int(v) for v in d["seeds"])
+    n_free, n_hit = (int(v) for v in d["n_cand"])
+    is_hit, fi, hi = d["is_hit"], d["free_idx"], d["hit_idx"]
+    free = knot_queries(n_free, seed_free, 0.0)
+    hit = knot_queries(n_hit, seed_hit, 1.0)
+    out = []
+    for f, h in zip(free, hit):
+        P = np.empty((len(is_hit), 12))
+        P[~is_hit] = f[fi]
+        P[is_hit] = h[hi]
+        out.append(P[:nq] if nq else P)
+    return tuple(out)
 
 
 # ---------------------------------------------------------------------------

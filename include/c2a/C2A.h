@@ -9,9 +9,12 @@
 // the demo does from the models' last_tri), updates each model's last_tri,
 // returns TOCFound, and for a collision writes the TOC poses and counts the
 // triangle pairs within 2 * distance + 0.001 (C2A.cpp:2411-2435).
-// Differences: no console output per call; res->R/T (the last relative
-// transform) and query_time_secs are not filled; contact pairs are counted,
-// not listed (cont_l stays empty).
+// After a collision, as in the reference, the contact query at the TOC pose
+// leaves mint = 1, abs_err = rel_err = 0, UpboundTOC = 1, res->R/T at the
+// TOC pose and the contact pairs in cont_l (C2A.cpp:1727-1775, push_front
+// order).  Differences: no console output per call; query_time_secs is the
+// wall time of the engine call.
+
 #ifndef C2A_COMPAT_C2A_H
 #define C2A_COMPAT_C2A_H
 
@@ -23,14 +26,35 @@
 
 enum C2A_Result { OK, TOCFound, CollisionFound, CollisionFree, CollisionNotFound };
 
-struct ContactF {   // contact pair record of the reference (not produced here)
+// Contact pair record (C2A.h:116-153, ListTri.h:43-61).  Feature types:
+// 1 vertex, 2 edge, 3 face; feature ids are AddTri's vertex indices;
+// triangle ids are build-order indices; P_A is in model 1's frame, P_B in
+// model 2's.  Ordered by distance.
+struct ContactF {
+  ContactF(int FeatureType_A, int FeatureType_B, int FeatureID_A[3], int FeatureID_B[3],
+           int TriangleID_A, int TriangleID_B, PQP_REAL P_A[3], PQP_REAL P_B[3],
+           PQP_REAL Distance)
+      : FeatureType_A(FeatureType_A), FeatureType_B(FeatureType_B),
+        TriangleID_A(TriangleID_A), TriangleID_B(TriangleID_B), Distance(Distance) {
+    IcI(this->FeatureID_A, FeatureID_A);
+    IcI(this->FeatureID_B, FeatureID_B);
+    VcV_L(this->P_A, P_A);
+    VcV_L(this->P_B, P_B);
+  }
+  void IcI(int a[3], int b[3]) { a[0] = b[0]; a[1] = b[1]; a[2] = b[2]; }
+  void VcV_L(PQP_REAL a[3], PQP_REAL b[3]) { a[0] = b[0]; a[1] = b[1]; a[2] = b[2]; }
+
   int FeatureType_A, FeatureType_B;
   int FeatureID_A[3], FeatureID_B[3];
   int TriangleID_A, TriangleID_B;
   PQP_REAL P_A[3], P_B[3];
   PQP_REAL Distance;
+
+  friend bool operator>(const ContactF& a, const ContactF& b) { return a.Distance > b.Distance; }
+  friend bool operator<(const ContactF& a, const ContactF& b) { return a.Distance < b.Distance; }
 };
 typedef std::list<ContactF> ContactFList;
+typedef std::list<ContactF>::iterator ContactFListIterator;
 
 struct C2A_TimeOfContactResult {
   int num_bv_tests;
@@ -125,7 +149,8 @@ int C2A_Collide(C2A_DistanceResult* result, PQP_REAL R1[3][3], PQP_REAL T1[3], C
                 PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,
                 PQP_REAL abs_err, int qsize = 2);
 
-// Counts the triangle pairs within `threshold` at the motions' current poses.
+// Counts and lists (res->cont_l) the triangle pairs within `threshold` at
+// the motions' current poses (C2A.cpp:1937-1966).
 PQP_REAL C2A_QueryContact(CInterpMotion* objmotion1, CInterpMotion* objmotion2,
                           C2A_TimeOfContactResult* res, C2A_Model* o1, C2A_Model* o2,
                           double threshold);

@@ -123,8 +123,18 @@ typedef struct {
   int32_t numCA;
   int32_t collisionfree;
   int32_t num_contact;
-  int32_t last_tri[2];
+  int32_t last_tri[2];      /* the models' last_tri after the query; for a
+                               query the translation-only branch ran, the
+                               result's last_triA/B (the last improving pair,
+                               C2A.cpp:1413-1414) -- that branch leaves the
+                               models' last_tri alone */
   int32_t flags;            /* C2A_FLAG_* */
+  /* res->R, res->T: model 2's pose in model 1's frame at the last CA step
+   * (C2A.cpp:1793-1796; the translation-only branch: at the advanced pose,
+   * C2A.cpp:1900-1904), and the step's error schedule and UpboundTOC */
+  double R[9];
+  double T[3];
+  double abs_err, rel_err, UpboundTOC;
 } c2a_result;
 
 enum {
@@ -192,10 +202,19 @@ int c2a_ccd_batch_noalloc(c2a_mesh *const *meshes, int32_t nmesh, const double *
                           const c2a_params *params, c2a_result *out, void *stream);
 
 /* Device-resident batches: inputs uploaded once, run repeatedly, results
- * fetched on demand (used to time the kernel with inputs already in HBM). */
+ * fetched on demand (used to time the kernel with inputs already in HBM).
+ * A batch keeps the device addresses of its meshes and, with the learned
+ * order, reads its scorer at every run: the meshes and the scorer must
+ * outlive the batch (destroy the batch first). */
 int c2a_batch_create(c2a_mesh *const *meshes, int32_t nmesh, const c2a_query *q, int64_t nq,
                      const c2a_params *params, c2a_batch **out);
 int c2a_batch_run(c2a_batch *b, void *stream);
+/* c2a_batch_run through a HIP graph: the first call captures the batch's
+ * launch sequence (the work-queue reset, a kernel, then the persistent CCD
+ * kernel, which runs every query's whole CA loop) and later calls replay it.
+ * Results are those of c2a_batch_run.  Not for batches with a scorer or
+ * sample logging. */
+int c2a_batch_run_graph(c2a_batch *b, void *stream);
 int c2a_batch_fetch(c2a_batch *b, c2a_result *out, void *stream);
 int c2a_batch_destroy(c2a_batch *b);
 /* Kernel-only device time of the last c2a_batch_run, in milliseconds,
@@ -234,6 +253,7 @@ typedef struct {
   int64_t bv_tests, tri_tests;
   int32_t last_tri[2];
   int32_t flags, pad;
+  double R[9], T[3];    /* res->R, res->T of the step (C2A.cpp:1793-1796) */
 } c2a_step_result;
 int c2a_ccd_step(c2a_mesh *const *meshes, int32_t nmesh, const c2a_step_query *q, int64_t nq,
                  c2a_step_result *out, void *stream);
@@ -243,6 +263,28 @@ int c2a_ccd_step(c2a_mesh *const *meshes, int32_t nmesh, const c2a_step_query *q
 int c2a_contact_count(c2a_mesh *const *meshes, int32_t nmesh, const double *poses /* nq x 2 x 12 */,
                       const int32_t *mesh_pairs /* nq x 2 */, const double *thresholds, int64_t nq,
                       int32_t *counts_out, void *stream);
+
+/* Contact list at explicit poses (C2A_QueryContact's cont_l,
+ * C2A.cpp:1525-1631, ContactF of C2A/C2A.h:116-153): every triangle pair
+ * within `threshold` whose closest features are classified, in the
+ * reference's visiting order (its std::list is built with push_front, so
+ * cont_l lists them in the reverse order).  out holds nq x cap records (the
+ * first min(count, cap) of query i at out + i * cap); counts_out (nq) gets
+ * every query's full count, as c2a_contact_count. */
+typedef struct {
+  int32_t tri[2];       /* TriangleID_A/B: triangle indices in build order */
+  int32_t type[2];      /* FeatureType_A/B: 1 vertex, 2 edge, 3 face */
+  int32_t fid[2][3];    /* FeatureID_A/B: vertex ids (AddTri i1, i2, i3) of the
+                           feature: the vertex; the edge's two vertices; the
+                           face's three; -1 where unused */
+  double pa[3];         /* P_A: closest point on triangle A, model 1's frame */
+  double pb[3];         /* P_B: closest point on triangle B, model 2's frame */
+  double distance;
+} c2a_contact;
+int c2a_contact_list(c2a_mesh *const *meshes, int32_t nmesh, const double *poses /* nq x 2 x 12 */,
+                     const int32_t *mesh_pairs /* nq x 2 */, const double *thresholds,
+                     int64_t nq, int32_t cap, c2a_contact *out, int32_t *counts_out,
+                     void *stream);
 
 /* Static distance query at explicit poses (C2A_Distance with qsize <= 2,
  * C2A/src/C2A_PQP.cpp:970-1057 -> C2ADistanceRecurse :481-620): the closest

@@ -1273,6 +1273,8 @@ void c2o_toc_step(const c2o_model *m1, const c2o_model *m2,
     res->rel_err = (res->numCA <= 2) ? 3 : 0.5;
   }
   res->mint = 1;
+  memcpy(res->R, cx.Rrel, sizeof(m3));      /* res->R, res->T (C2A.cpp:1793-1796) */
+  memcpy(res->T, cx.Trel, sizeof(cx.Trel));
   toc_recurse(&cx, R, T, 0, 0);
   if (upd1) upd1->last_tri = cx.last1;
   if (upd2) upd2->last_tri = cx.last2;
@@ -1500,12 +1502,16 @@ static double query_toc_translation(c2o_model *m1, c2o_model *m2, c2o_motion *mo
   mtxv(Tr, R1, Ttmp);
   res->distance = tri_distance(Rr, Tr, &m1->tris[cx.la], &m2->tris[cx.lb], res->p1, res->p2,
                                NULL, NULL, NULL, NULL);
+  memcpy(res->R, Rr, sizeof(m3));           /* res->R, res->T (C2A.cpp:1902-1904) */
+  memcpy(res->T, Tr, sizeof(Tr));
   res->toc = res->mint;
   res->collisionfree = res->toc >= 1.0 ? 1 : 0;
   res->numCA = 0;
   if (!res->collisionfree) c2o_motion_integrate(mo1, res->toc, res->R_toc, res->T_toc, NULL);
-  res->out_last_triA = m1->last_tri;
-  res->out_last_triB = m2->last_tri;
+  /* the models' last_tri stay as they were; res->last_triA/B become the last
+   * improving pair (C2A.cpp:1413-1414), which out_last_tri* report here */
+  res->out_last_triA = cx.la;
+  res->out_last_triB = cx.lb;
   return res->toc;
 }
 

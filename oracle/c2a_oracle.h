@@ -88,8 +88,13 @@ typedef struct {
   long long num_bv_tests;
   long long num_tri_tests;
   int last_triA, last_triB;   /* seed pair (caller-owned in the reference) */
-  int out_last_triA, out_last_triB; /* model last_tri after the query */
+  int out_last_triA, out_last_triB; /* model last_tri after the query
+                                       (translation-only branch: the
+                                       result's last_triA/B, the models'
+                                       last_tri being left alone) */
   int flags;                  /* C2O_FLAG_* */
+  double R[3][3], T[3];       /* res->R, res->T: model 2 in model 1's frame at
+                                 the last step (C2A.cpp:1793-1796) */
 } c2o_result;
 
 enum {

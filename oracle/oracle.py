@@ -44,7 +44,8 @@ class Result(C.Structure):
                 ("numCA", C.c_int), ("collisionfree", C.c_int), ("num_contact", C.c_int),
                 ("num_bv_tests", C.c_longlong), ("num_tri_tests", C.c_longlong),
                 ("last_triA", C.c_int), ("last_triB", C.c_int),
-                ("out_last_triA", C.c_int), ("out_last_triB", C.c_int), ("flags", C.c_int)]
+                ("out_last_triA", C.c_int), ("out_last_triB", C.c_int), ("flags", C.c_int),
+                ("R", m33), ("T", d3)]
 
 
 class Contact(C.Structure):

@@ -30,6 +30,19 @@ int main(int argc, char** argv) {
   std::vector<double> ts((size_t)nt);
   if (fread(ts.data(), 8, ts.size(), f) != ts.size()) return 3;
   fclose(f);
+  {
+    // ContactF as C2A.h:116-153 / ListTri.h:43-61 declare it
+    int fa[3] = {1, 2, 3}, fb[3] = {4, -1, -1};
+    PQP_REAL pa[3] = {0.5, 0, 0}, pb[3] = {0, 0.25, 0};
+    ContactF c1(3, 1, fa, fb, 10, 20, pa, pb, 0.75), c2(2, 2, fb, fa, 11, 21, pb, pa, 0.5);
+    if (!(c2 < c1) || !(c1 > c2) || c1 < c2) return 6;
+    if (c1.FeatureID_A[2] != 3 || c1.TriangleID_B != 20 || c1.P_B[1] != 0.25) return 6;
+    ContactFList l;
+    l.push_front(c1);
+    l.push_front(c2);
+    l.sort();
+    if (l.front().Distance != 0.5) return 6;
+  }
   C2A_Model m;
   if (m.BeginModel() != PQP_OK) return 4;
   for (int i = 0; i < n; i++) m.AddTri(&t[9 * i], &t[9 * i + 3], &t[9 * i + 6], 1000 + i);

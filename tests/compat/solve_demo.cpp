@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <cmath>
 #include <vector>
 
 #include "c2a/C2A.h"
@@ -42,10 +43,13 @@ static void put_transform(const Transform& t, double* o) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 3) {
-    fprintf(stderr, "usage: solve_demo input.bin output.bin\n");
+  if (argc != 3 && argc != 4) {
+    fprintf(stderr, "usage: solve_demo input.bin output.bin [reuse]\n");
     return 2;
   }
+  // reuse: dres (and its last_triA/B) carries over from call to call instead
+  // of being seeded from the models' last_tri before each call
+  const bool reuse = argc == 4;
   FILE* f = fopen(argv[1], "rb");
   if (!f) return 2;
   C2A_Model obj1, obj2;
@@ -68,19 +72,43 @@ int main(int argc, char** argv) {
     trans1.Identity();
     PQP_REAL toc = 0;
     int nItr = 0, nContact = 0;
-    dres.last_triA = obj1.last_tri;
-    dres.last_triB = obj2.last_tri;
+    if (!reuse || q == 0) {
+      dres.last_triA = obj1.last_tri;
+      dres.last_triB = obj2.last_tri;
+    }
     C2A_Result r = C2A_Solve(&trans00, &trans01, &obj1, &trans10, &trans11, &obj2, trans0, trans1,
                              toc, nItr, nContact, 0.0, dres);
     if (r != TOCFound) return 4;
+    if (!dres.collisionfree) {
+      // the contact query at the TOC pose (C2A.cpp:2433-2435, 1727-1775):
+      // one ContactF per counted pair, each within 2 d + 0.001, mint 1,
+      // zero error schedule, res->R/T at the TOC pose
+      if ((int)dres.cont_l.size() != nContact) return 5;
+      ContactFList sorted = dres.cont_l;
+      sorted.sort();   // ContactF::operator<
+      double prev = -1.0;
+      for (ContactFListIterator it = sorted.begin(); it != sorted.end(); ++it) {
+        if (it->Distance < prev || it->Distance > 2 * dres.distance + 0.001) return 6;
+        if (it->FeatureType_A < 1 || it->FeatureType_A > 3) return 6;
+        prev = it->Distance;
+      }
+      if (dres.mint != 1.0 || dres.abs_err != 0.0 || dres.rel_err != 0.0) return 7;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          double v = 0;
+          for (int k = 0; k < 3; k++) v += trans0.Rotation()[k][i] * trans1.Rotation()[k][j];
+          if (std::fabs(v - dres.R[i][j]) > 1e-12) return 8;
+        }
+    }
     double d2[2] = {toc, dres.distance};
-    int i5[5] = {nItr, nContact, dres.collisionfree ? 1 : 0, obj1.tri_index(obj1.last_tri),
-                 obj2.tri_index(obj2.last_tri)};
+    int i5[7] = {nItr, nContact, dres.collisionfree ? 1 : 0, obj1.tri_index(obj1.last_tri),
+                 obj2.tri_index(obj2.last_tri), obj1.tri_index(dres.last_triA),
+                 obj2.tri_index(dres.last_triB)};
     double tr[24];
     put_transform(trans0, tr);
     put_transform(trans1, tr + 12);
     fwrite(d2, 8, 2, o);
-    fwrite(i5, 4, 5, o);
+    fwrite(i5, 4, 7, o);
     fwrite(tr, 8, 24, o);
   }
   fclose(o);

@@ -29,10 +29,10 @@ def _binary(name):
     return path
 
 
-def _run(name, payload, tmp_path):
+def _run(name, payload, tmp_path, *extra):
     inp, out = tmp_path / (name + ".in"), tmp_path / (name + ".out")
     inp.write_bytes(payload)
-    subprocess.run([_binary(name), str(inp), str(out)], check=True, timeout=300)
+    subprocess.run([_binary(name), str(inp), str(out)] + list(extra), check=True, timeout=300)
     return out.read_bytes()
 
 
@@ -89,7 +89,7 @@ def test_solve_demo_matches_oracle(tmp_path):
                np.ascontiguousarray(poses).tobytes())
     print("solve_demo: %d queries" % nq, flush=True)
     b = _run("solve_demo", payload, tmp_path)
-    rec = np.dtype([("td", "<f8", 2), ("i5", "<i4", 5), ("tr", "<f8", 24)])
+    rec = np.dtype([("td", "<f8", 2), ("i5", "<i4", 7), ("tr", "<f8", 24)])
     got = np.frombuffer(b, rec)
     assert len(got) == nq
     m1, m2 = O.OModel(t9), O.OModel(t9)
@@ -124,7 +124,7 @@ def test_reference_demo_bunny_matches_oracle(tmp_path):
                np.ascontiguousarray(poses).tobytes())
     print("solve_demo on the bunny demo: %d queries" % nq, flush=True)
     b = _run("solve_demo", payload, tmp_path)
-    rec = np.dtype([("td", "<f8", 2), ("i5", "<i4", 5), ("tr", "<f8", 24)])
+    rec = np.dtype([("td", "<f8", 2), ("i5", "<i4", 7), ("tr", "<f8", 24)])
     got = np.frombuffer(b, rec)
     assert len(got) == nq
     m1, m2 = O.OModel(t9), O.OModel(t9)
@@ -179,3 +179,37 @@ def test_distance_sequence_matches_oracle(tmp_path):
         Rrel = R1.T @ R2
         assert np.allclose(g["rt"][:9], Rrel.reshape(-1), atol=1e-12), i
         assert np.allclose(g["rt"][9:], R1.T @ (T2 - T1), atol=1e-12), i
+
+
+@pytest.mark.gpu
+def test_solve_reused_result_translation_only(tmp_path):
+    """C2A_Solve with one C2A_TimeOfContactResult reused across calls on
+    translation-only motions: the translation branch leaves the models'
+    last_tri alone and moves dres.last_triA/B to its last improving pair
+    (C2A.cpp:1413-1414), which seeds the next call."""
+    tris, _ = W.torus_knot(n_seg=96, n_ring=10)
+    t9 = np.ascontiguousarray(tris.reshape(-1, 9))
+    nq = 40
+    P = [p.copy() for p in W.knot_queries(nq, 909, collide_frac=0.5)]
+    P[1][:, :9] = P[0][:, :9]      # end rotation = start rotation: pure translation
+    P[3][:, :9] = P[2][:, :9]
+    poses = np.stack(P, axis=1)
+    payload = (_i32(len(t9)) + t9.tobytes() + _i32(len(t9)) + t9.tobytes() + _i32(nq) +
+               np.ascontiguousarray(poses).tobytes())
+    b = _run("solve_demo", payload, tmp_path, "reuse")
+    rec = np.dtype([("td", "<f8", 2), ("i5", "<i4", 7), ("tr", "<f8", 24)])
+    got = np.frombuffer(b, rec)
+    m1, m2 = O.OModel(t9), O.OModel(t9)
+    seed = (0, 0)
+    for i in range(nq):
+        p = poses[i]
+        res = O.query_toc(m1, m2, (p[0, :9].reshape(3, 3), p[0, 9:]),
+                          (p[1, :9].reshape(3, 3), p[1, 9:]),
+                          (p[2, :9].reshape(3, 3), p[2, 9:]),
+                          (p[3, :9].reshape(3, 3), p[3, 9:]), 1e-4, 1e-4, seed=seed)
+        g = got[i]
+        assert res.flags & O.FLAG_TRANSLATION_ONLY, i
+        assert g["td"][0] == res.toc, i
+        assert (g["i5"][3], g["i5"][4]) == (0, 0), i          # models untouched
+        assert (g["i5"][5], g["i5"][6]) == (res.out_last_triA, res.out_last_triB), i
+        seed = (res.out_last_triA, res.out_last_triB)

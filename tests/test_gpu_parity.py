@@ -29,6 +29,10 @@ def assert_same(ref, got, label):
                                  % (label, f, len(bad), bad[:5], a[bad[:5]], b[bad[:5]]))
     assert np.array_equal(ref["out_last_triA"], got["last_tri"][:, 0]), label
     assert np.array_equal(ref["out_last_triB"], got["last_tri"][:, 1]), label
+    assert np.array_equal(ref["R"].reshape(-1, 9), got["R"]), label
+    assert np.array_equal(ref["T"], got["T"]), label
+    for f in ("abs_err", "rel_err", "UpboundTOC"):
+        assert np.array_equal(ref[f], got[f]), (label, f)
     nc = ref["collisionfree"] == 0
     assert np.array_equal(ref["R_toc"].reshape(-1, 9)[nc], got["R_toc"][nc]), label
     assert np.array_equal(ref["T_toc"][nc], got["T_toc"][nc]), label
@@ -78,3 +82,18 @@ def test_traversal_knobs_bit_exact(knot_pair, knob):
     ref = O.batch_toc(m1, m2, *P, 1e-3, 1e-4, seeds=np.zeros((nq, 2), np.int32),
                       with_contacts=True, nthreads=8)
     assert_same(ref, got, knob)
+
+
+def test_knot_batch_4096_full_bit_exact(knot_pair):
+    """The whole 4096-query knot batch (about 30% colliding) against the
+    oracle, every field, with C2A_Solve's contact count."""
+    t9, meshes, m1, m2 = knot_pair
+    P = W.knot_batch_queries()
+    nq = len(P[0])
+    print("knot batch 4096 full", flush=True)
+    got = G.ccd_batch(meshes, G.make_queries(*P), G.default_params(tol_d=1e-3, tol_t=1e-4))
+    ref = O.batch_toc(m1, m2, *P, 1e-3, 1e-4, seeds=np.zeros((nq, 2), np.int32),
+                      with_contacts=True, nthreads=16)
+    assert_same(ref, got, "knot 4096")
+    frac = float((got["collisionfree"] == 0).mean())
+    assert 0.25 < frac < 0.35, frac

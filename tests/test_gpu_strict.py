@@ -100,7 +100,7 @@ def _sampled_contact_free(m1, m2, P, res, idx, nsamp=8):
 def test_strict_knot_4096_never_past_contact(knot, scorer, order):
     t9, meshes, m1, m2 = knot
     cfg = W.CONFIG_KNOT_BATCH
-    P = W.knot_queries(cfg["nq"], cfg["seed"], collide_frac=cfg["collide_frac"])
+    P = W.knot_batch_queries()
     print("strict knot 4096 order=%s" % order, flush=True)
     kw = dict(order=G.ORDER_LEARNED, scorer=scorer) if order == "learned" else {}
     res = G.ccd_batch(meshes, G.make_queries(*P),
@@ -109,6 +109,23 @@ def test_strict_knot_4096_never_past_contact(knot, scorer, order):
     assert len(in_contact_at_toc(res)) == 0, in_contact_at_toc(res)[:10]
     coll = np.nonzero((res["collisionfree"] == 0) & (res["toc"] > 0))[0]
     assert len(coll) > 1000
+    bad = _sampled_contact_free(m1, m2, P, res, coll)
+    assert not bad, bad[:10]
+
+
+def test_strict_old_knot_stream_never_past_contact(knot):
+    """The first knot stream (knot_queries(4096, 20261, 0.3)), on which the
+    reference semantics reach contact on 4 queries: strict mode on none."""
+    t9, meshes, m1, m2 = knot
+    P = W.knot_queries(4096, W.CONFIG_KNOT_BATCH["seed"], 0.3)
+    print("strict knot stream 20261", flush=True)
+    ref = G.ccd_batch(meshes, G.make_queries(*P),
+                      G.default_params(tol_d=1e-3, tol_t=1e-4, with_contacts=0))
+    res = G.ccd_batch(meshes, G.make_queries(*P),
+                      G.default_params(tol_d=1e-3, tol_t=1e-4, with_contacts=0, strict=1))
+    assert len(in_contact_at_toc(ref)) >= 1
+    assert len(in_contact_at_toc(res)) == 0, in_contact_at_toc(res)[:10]
+    coll = np.nonzero((res["collisionfree"] == 0) & (res["toc"] > 0))[0]
     bad = _sampled_contact_free(m1, m2, P, res, coll)
     assert not bad, bad[:10]
 

@@ -188,6 +188,8 @@ def test_two_triangle_step_from_far_seed():
     res, M1, M2 = step((0, 0))
     assert res.distance == 6.0
     assert res.mint == 6.0 / 16.0
+    # res->R, res->T: model 2's pose in model 1's frame (C2A.cpp:1793-1796)
+    assert np.array_equal(np.array(res.R), np.eye(3)) and list(res.T) == [10.0, 0.0, 0.0]
     assert (res.num_bv_tests, res.num_tri_tests) == (4, 1)
     assert (M1.last_tri, M2.last_tri) == (1, 0)
     assert (res.abs_err, res.rel_err) == (0.0, 0.0)

@@ -133,15 +133,14 @@ def _exact_distance(m1, m2, P, i, t):
 
 
 def test_strict_never_reaches_contact_knot():
-    """Knot batch (a 768-query slice of the 4096-query workload): in strict
-    mode no colliding query's TOC pose, nor any of 8 poses sampled before it,
-    is in contact (exact distance > 0); the reference order reaches contact
-    on this slice (query 608 of the batch)."""
+    """A 768-query slice of the seeded knot stream knot_queries(4096, 20261,
+    0.3): in strict mode no colliding query's TOC pose, nor any of 8 poses
+    sampled before it, is in contact (exact distance > 0); the reference
+    semantics reach contact on this slice (query 608 of the stream)."""
     tris, _ = W.torus_knot()
     t9 = tris.reshape(-1, 9)
     m1, m2 = O.OModel(t9), O.OModel(t9)
-    cfg = W.CONFIG_KNOT_BATCH
-    P = W.knot_queries(cfg["nq"], cfg["seed"], collide_frac=cfg["collide_frac"])
+    P = W.knot_queries(4096, W.CONFIG_KNOT_BATCH["seed"], collide_frac=0.3)
     P = [p[256:1024] for p in P]
     nq = len(P[0])
     seeds = np.zeros((nq, 2), np.int32)
