@@ -14,7 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libc2a_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "c2a_amd.h")
 
-OK, ERR_BAD_ARG, ERR_OOM, ERR_HIP, ERR_NO_DEVICE, ERR_UNSUPPORTED, ERR_EMPTY_MODEL = range(7)
+(OK, ERR_BAD_ARG, ERR_OOM, ERR_HIP, ERR_NO_DEVICE, ERR_UNSUPPORTED, ERR_EMPTY_MODEL,
+ ERR_CACHE) = range(8)
 FLAG_ITER_CAP, FLAG_TRANSLATION_ONLY, FLAG_STACK_OVERFLOW, FLAG_BAD_INPUT = 1, 2, 4, 8
 NOALLOC_SCRATCH_MIB, NOALLOC_MAX_MESHES = 128, 32
 ORDER_PLAIN, ORDER_LEARNED = 0, 1
@@ -85,6 +86,11 @@ DISTANCE_DTYPE = np.dtype([("distance", "f8"), ("p1", "f8", 3), ("p2", "f8", 3),
                            ("bv_tests", "i8"), ("tri_tests", "i8"), ("last_tri", "i4", 2),
                            ("flags", "i4"), ("pad", "i4")])
 
+# c2a_static_result
+STATIC_DTYPE = np.dtype([("distance", "f8"), ("p1", "f8", 3), ("p2", "f8", 3),
+                         ("bv_tests", "i8"), ("tri_tests", "i8"), ("tri", "i4", 2),
+                         ("found", "i4"), ("flags", "i4")])
+
 _lib = None
 
 # Every extern "C" entry point of include/c2a_amd.h.
@@ -97,7 +103,9 @@ EXPORTS = ["c2a_params_default", "c2a_mesh_create", "c2a_mesh_destroy", "c2a_mes
            "c2a_scorer_set_params", "c2a_scorer_get_params", "c2a_scorer_infer",
            "c2a_scorer_grad", "c2a_scorer_train", "c2a_scorer_save", "c2a_scorer_load",
            "c2a_batch_samples", "c2a_ccd_batch_noalloc", "c2a_contact_list",
-           "c2a_batch_run_graph"]
+           "c2a_batch_run_graph", "c2a_distance_queue", "c2a_distance_contact_space",
+           "c2a_tolerance", "c2a_collide_pairs", "c2a_mesh_create_device", "c2a_mesh_save",
+           "c2a_mesh_load"]
 
 
 def lib():
@@ -115,6 +123,9 @@ def lib():
     L.c2a_params_default.restype = None
     L.c2a_mesh_create.argtypes = [dp, ip, ip, i64, C.POINTER(vp)]
     L.c2a_mesh_destroy.argtypes = [vp]
+    L.c2a_mesh_create_device.argtypes = [dp, ip, ip, i64, vp, C.POINTER(vp)]
+    L.c2a_mesh_save.argtypes = [vp, C.c_char_p]
+    L.c2a_mesh_load.argtypes = [C.c_char_p, dp, ip, ip, i64, C.POINTER(vp)]
     L.c2a_mesh_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), ip]
     L.c2a_mesh_tri_ids.argtypes = [vp, ip]
     L.c2a_mesh_export.argtypes = [vp, dp, dp]
@@ -138,6 +149,13 @@ def lib():
     L.c2a_distance.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, ip, C.c_double, C.c_double,
                                i64, vp, vp]
     L.c2a_collide_distance.argtypes = L.c2a_distance.argtypes
+    L.c2a_distance_queue.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, ip, C.c_double,
+                                     C.c_double, C.c_int32, i64, vp, vp]
+    L.c2a_distance_contact_space.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, ip, dp, i64, vp,
+                                             vp]
+    L.c2a_tolerance.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, dp, C.c_int32, i64, vp, vp]
+    L.c2a_collide_pairs.argtypes = [C.POINTER(vp), C.c_int32, dp, ip, C.c_int32, i64,
+                                    C.c_int32, ip, vp, vp]
     L.c2a_ccd_batch_noalloc.argtypes = [C.POINTER(vp), C.c_int32, vp, vp, vp, i64,
                                         C.POINTER(Params), vp, vp]
     L.c2a_motion_prepare.argtypes = [dp, dp, dp]
@@ -181,16 +199,34 @@ def default_params(**kw):
 
 
 class Mesh:
-    """A triangle soup with its RSS hierarchy (C2A_Model after EndModel)."""
+    """A triangle soup with its RSS hierarchy (C2A_Model after EndModel).
 
-    def __init__(self, tris, ids=None, vidx=None):
+    device=True builds the hierarchy on the GPU (the same nodes as the host
+    build).  cache=path reuses a hierarchy file written for the same
+    triangles, ids and vertex indices, and otherwise builds and writes one."""
+
+    def __init__(self, tris, ids=None, vidx=None, device=False, cache=None):
         tris = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
         self.ntri = tris.shape[0]
         ida = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
         vda = None if vidx is None else np.ascontiguousarray(vidx, dtype=np.int32).reshape(-1)
+        args = (_dp(tris), None if ida is None else _ip(ida), None if vda is None else _ip(vda),
+                self.ntri)
         h = C.c_void_p()
-        _check(lib().c2a_mesh_create(_dp(tris), None if ida is None else _ip(ida),
-                                     None if vda is None else _ip(vda), self.ntri, C.byref(h)))
+        self.from_cache = False
+        if cache is not None and os.path.exists(cache):
+            rc = lib().c2a_mesh_load(os.fsencode(cache), *args, C.byref(h))
+            if rc == OK:
+                self.from_cache = True
+            elif rc != ERR_CACHE:
+                _check(rc)
+        if not self.from_cache:
+            if device:
+                _check(lib().c2a_mesh_create_device(*args, None, C.byref(h)))
+            else:
+                _check(lib().c2a_mesh_create(*args, C.byref(h)))
+            if cache is not None:
+                _check(lib().c2a_mesh_save(h, os.fsencode(cache)))
         self.handle = h
         n, b, d = C.c_int64(), C.c_int64(), C.c_int32()
         _check(lib().c2a_mesh_info(h, C.byref(n), C.byref(b), C.byref(d)))
@@ -210,6 +246,10 @@ class Mesh:
         tris = np.zeros((self.ntri, 9))
         _check(lib().c2a_mesh_export(self.handle, _dp(nodes), _dp(tris)))
         return nodes, tris
+
+    def save(self, path):
+        """Writes the hierarchy file (c2a_mesh_save)."""
+        _check(lib().c2a_mesh_save(self.handle, os.fsencode(path)))
 
     def tri_ids(self):
         ids = np.zeros(self.ntri, dtype=np.int32)
@@ -347,6 +387,66 @@ def distance(meshes, poses, mesh_pairs, seed_tris=None, rel_err=0.0, abs_err=0.0
              _ip(seeds) if seeds is not None else None, rel_err, abs_err, nq, out.ctypes.data,
              None))
     return out
+
+
+def _static_args(poses, mesh_pairs):
+    poses = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 24)
+    nq = len(poses)
+    return poses, nq, np.ascontiguousarray(mesh_pairs, dtype=np.int32).reshape(nq, 2)
+
+
+def _per_query(v, nq):
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (nq,)))
+
+
+def distance_queue(meshes, poses, mesh_pairs, qsize, seed_tris=None, rel_err=0.0,
+                   abs_err=0.0):
+    """C2A_Distance with a queue of qsize tests (> 2: best-first search);
+    returns a DISTANCE_DTYPE array."""
+    poses, nq, mp = _static_args(poses, mesh_pairs)
+    seeds = None
+    if seed_tris is not None:
+        seeds = np.ascontiguousarray(seed_tris, dtype=np.int32).reshape(nq, 2)
+    out = np.zeros(nq, dtype=DISTANCE_DTYPE)
+    _check(lib().c2a_distance_queue(_mesh_table(meshes), len(meshes), _dp(poses), _ip(mp),
+                                    _ip(seeds) if seeds is not None else None, rel_err, abs_err,
+                                    int(qsize), nq, out.ctypes.data, None))
+    return out
+
+
+def distance_contact_space(meshes, poses, mesh_pairs, upper_bounds, seed_tris=None):
+    """C2A_Distance_At_ContactSpace; returns a STATIC_DTYPE array."""
+    poses, nq, mp = _static_args(poses, mesh_pairs)
+    up = _per_query(upper_bounds, nq)
+    seeds = None
+    if seed_tris is not None:
+        seeds = np.ascontiguousarray(seed_tris, dtype=np.int32).reshape(nq, 2)
+    out = np.zeros(nq, dtype=STATIC_DTYPE)
+    _check(lib().c2a_distance_contact_space(_mesh_table(meshes), len(meshes), _dp(poses),
+                                            _ip(mp), _ip(seeds) if seeds is not None else None,
+                                            _dp(up), nq, out.ctypes.data, None))
+    return out
+
+
+def tolerance(meshes, poses, mesh_pairs, tolerances, qsize=2):
+    """C2A_Tolerance; returns a STATIC_DTYPE array (found = closer)."""
+    poses, nq, mp = _static_args(poses, mesh_pairs)
+    tol = _per_query(tolerances, nq)
+    out = np.zeros(nq, dtype=STATIC_DTYPE)
+    _check(lib().c2a_tolerance(_mesh_table(meshes), len(meshes), _dp(poses), _ip(mp), _dp(tol),
+                               int(qsize), nq, out.ctypes.data, None))
+    return out
+
+
+def collide_pairs(meshes, poses, mesh_pairs, flag=1, cap=1024):
+    """C2A_Collide's PQP_CollideResult form; returns (STATIC_DTYPE array,
+    pairs (nq, cap, 2) of build-order triangles, -1 past each count)."""
+    poses, nq, mp = _static_args(poses, mesh_pairs)
+    out = np.zeros(nq, dtype=STATIC_DTYPE)
+    pairs = np.full((nq, cap, 2), -1, dtype=np.int32)
+    _check(lib().c2a_collide_pairs(_mesh_table(meshes), len(meshes), _dp(poses), _ip(mp),
+                                   int(flag), nq, int(cap), _ip(pairs), out.ctypes.data, None))
+    return out, pairs
 
 
 class Scorer:

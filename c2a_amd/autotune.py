@@ -10,9 +10,12 @@ and, for the bucket above 131,072 triangles, the grazing batch between two
   miss_depth  depth of the first round below a replayed pair the step must
               expand further than the last one
   replay_log  expansions logged per CA step for the next step's replay
+  ext_levels  levels the replay computes ahead below replayed pairs that
+              newly pass the descend test
 
-(depth and miss depth first at the bucket's default log, then the log at
-the best pair), median of three kernel timings after a warm-up.  Every
+(depth and miss depth first at the bucket's default log without extension,
+then the log at the best pair, then the extension levels at the best
+triple), median of three kernel timings after a warm-up.  Every
 setting gives the same results bit for bit; only the time differs.  The
 chosen knobs go to tuning.json, which c2a_amd/build.py compiles into the
 library: c2a_batch_create applies them per query when spec_depth is 0.
@@ -48,9 +51,9 @@ def default_log(max_ntri):
     return cap
 
 
-def time_config(meshes, queries, tol_d, tol_t, depth, miss, log, reps=3):
+def time_config(meshes, queries, tol_d, tol_t, depth, miss, log, ext=0, reps=3):
     p = G.default_params(tol_d=tol_d, tol_t=tol_t, with_contacts=0, spec_depth=depth,
-                         miss_depth=miss, replay_log=log)
+                         miss_depth=miss, replay_log=log, ext_levels=ext)
     b = G.Batch(meshes, queries, p)
     b.run()
     ms = []
@@ -60,7 +63,7 @@ def time_config(meshes, queries, tol_d, tol_t, depth, miss, log, reps=3):
     return float(np.median(ms)), b.fetch()
 
 
-def tune_bucket(meshes, queries, tol_d, tol_t, max_ntri, depths, misses, logs, log):
+def tune_bucket(meshes, queries, tol_d, tol_t, max_ntri, depths, misses, logs, exts, log):
     base_log = default_log(max_ntri)
     trials = []
     best = None
@@ -74,7 +77,8 @@ def tune_bucket(meshes, queries, tol_d, tol_t, max_ntri, depths, misses, logs, l
                 ref = res
             elif not np.array_equal(ref["toc"], res["toc"]):
                 raise SystemExit("autotune: results differ between settings")
-            trials.append({"spec_depth": d, "miss_depth": m, "replay_log": base_log, "ms": ms})
+            trials.append({"spec_depth": d, "miss_depth": m, "replay_log": base_log,
+                           "ext_levels": 0, "ms": ms})
             log("  depth %d miss %d log %d: %.2f ms" % (d, m, base_log, ms))
             if best is None or ms < best["ms"]:
                 best = trials[-1]
@@ -86,8 +90,20 @@ def tune_bucket(meshes, queries, tol_d, tol_t, max_ntri, depths, misses, logs, l
         if not np.array_equal(ref["toc"], res["toc"]):
             raise SystemExit("autotune: results differ between settings")
         trials.append({"spec_depth": best["spec_depth"], "miss_depth": best["miss_depth"],
-                       "replay_log": lg, "ms": ms})
+                       "replay_log": lg, "ext_levels": 0, "ms": ms})
         log("  depth %d miss %d log %d: %.2f ms" % (best["spec_depth"], best["miss_depth"], lg, ms))
+        if ms < best["ms"]:
+            best = trials[-1]
+    for ext in exts:
+        if ext == 0:
+            continue
+        bd, bm, bl = best["spec_depth"], best["miss_depth"], best["replay_log"]
+        ms, res = time_config(meshes, queries, tol_d, tol_t, bd, bm, bl, ext)
+        if not np.array_equal(ref["toc"], res["toc"]):
+            raise SystemExit("autotune: results differ between settings")
+        trials.append({"spec_depth": bd, "miss_depth": bm, "replay_log": bl, "ext_levels": ext,
+                       "ms": ms})
+        log("  depth %d miss %d log %d ext %d: %.2f ms" % (bd, bm, bl, ext, ms))
         if ms < best["ms"]:
             best = trials[-1]
     return best, trials
@@ -99,11 +115,13 @@ def main():
     ap.add_argument("--depths", default="3,4,5,6")
     ap.add_argument("--misses", default="1,2,3")
     ap.add_argument("--logs", default="2048,8192,32768")
+    ap.add_argument("--exts", default="0,1,2")
     ap.add_argument("--no-grazing", action="store_true")
     a = ap.parse_args()
     depths = [int(v) for v in a.depths.split(",")]
     misses = [int(v) for v in a.misses.split(",")]
     logs = [int(v) for v in a.logs.split(",")]
+    exts = [int(v) for v in a.exts.split(",")]
     t0 = time.time()
 
     def log(msg):
@@ -132,18 +150,20 @@ def main():
             wl = ("grazing", gm, G.make_queries(*P), g["nq"], len(t9), g["tol_d"], g["tol_t"])
         if wl is None:
             rows.append({"max_ntri": max_ntri, "spec_depth": 5, "miss_depth": 2,
-                         "replay_log": default_log(max_ntri)})
+                         "replay_log": default_log(max_ntri), "ext_levels": 0})
             log("bucket <= %d: no queries, defaults" % max_ntri)
             continue
         name, ms_, q, n, mx, td, tt = wl
         log("bucket <= %d: %d %s queries (largest mesh %d triangles)" % (max_ntri, n, name, mx))
-        best, trials = tune_bucket(ms_, q, td, tt, mx, depths, misses, logs, log)
+        best, trials = tune_bucket(ms_, q, td, tt, mx, depths, misses, logs, exts, log)
         rows.append({"max_ntri": max_ntri, "spec_depth": best["spec_depth"],
-                     "miss_depth": best["miss_depth"], "replay_log": best["replay_log"]})
+                     "miss_depth": best["miss_depth"], "replay_log": best["replay_log"],
+                     "ext_levels": best["ext_levels"]})
         measured[str(max_ntri)] = {"workload": name, "queries": n, "best_ms": best["ms"],
                                    "trials": trials}
-        log("  -> depth %d miss %d log %d (%.2f ms)" % (best["spec_depth"], best["miss_depth"],
-                                                     best["replay_log"], best["ms"]))
+        log("  -> depth %d miss %d log %d ext %d (%.2f ms)"
+            % (best["spec_depth"], best["miss_depth"], best["replay_log"], best["ext_levels"],
+               best["ms"]))
     doc = json.load(open(DEFAULT_TABLE)) if os.path.exists(DEFAULT_TABLE) else {}
     doc["buckets"] = rows
     doc["measured"] = measured

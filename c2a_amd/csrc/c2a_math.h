@@ -753,4 +753,58 @@ C2A_HD double tri_distance(const double* R, const double* T, const double* t1, c
   return tri_dist(p, q, t1, b, feat);
 }
 
+// PQP's TriContact (declared at C2A_PQP.cpp:441, called by CollideRecurse
+// :831; PQP itself is not vendored): the separating-axis test over the two
+// face normals, the nine edge-edge cross products and the six in-plane edge
+// normals, vertices taken relative to P1.  1 when the projections overlap
+// on every axis (touching counts as contact).
+C2A_HD bool project6(const double* ax, const double* p1, const double* p2, const double* p3,
+                     const double* q1, const double* q2, const double* q3) {
+  const double P1 = dot3(ax, p1), P2 = dot3(ax, p2), P3 = dot3(ax, p3);
+  const double Q1 = dot3(ax, q1), Q2 = dot3(ax, q2), Q3 = dot3(ax, q3);
+  double mx1 = P1 > P2 ? P1 : P2, mn1 = P1 < P2 ? P1 : P2;
+  mx1 = mx1 > P3 ? mx1 : P3;
+  mn1 = mn1 < P3 ? mn1 : P3;
+  double mx2 = Q1 > Q2 ? Q1 : Q2, mn2 = Q1 < Q2 ? Q1 : Q2;
+  mx2 = mx2 > Q3 ? mx2 : Q3;
+  mn2 = mn2 < Q3 ? mn2 : Q3;
+  return !(mn1 > mx2) && !(mn2 > mx1);
+}
+
+C2A_HD bool tri_contact(const double* P1, const double* P2, const double* P3, const double* Q1,
+                        const double* Q2, const double* Q3) {
+  const double p1[3] = {0.0, 0.0, 0.0};
+  double p2[3], p3[3], q1[3], q2[3], q3[3];
+  double e[3][3], f[3][3], n1[3], m1[3], g[3][3], h[3][3], ef[3];
+  sub3(p2, P2, P1);
+  sub3(p3, P3, P1);
+  sub3(q1, Q1, P1);
+  sub3(q2, Q2, P1);
+  sub3(q3, Q3, P1);
+  sub3(e[0], p2, p1);
+  sub3(e[1], p3, p2);
+  sub3(e[2], p1, p3);
+  sub3(f[0], q2, q1);
+  sub3(f[1], q3, q2);
+  sub3(f[2], q1, q3);
+  cross3(n1, e[0], e[1]);
+  cross3(m1, f[0], f[1]);
+  for (int i = 0; i < 3; i++) {
+    cross3(g[i], e[i], n1);
+    cross3(h[i], f[i], m1);
+  }
+  if (!project6(n1, p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6(m1, p1, p2, p3, q1, q2, q3)) return false;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      cross3(ef, e[i], f[j]);
+      if (!project6(ef, p1, p2, p3, q1, q2, q3)) return false;
+    }
+  for (int i = 0; i < 3; i++) {
+    if (!project6(g[i], p1, p2, p3, q1, q2, q3)) return false;
+    if (!project6(h[i], p1, p2, p3, q1, q2, q3)) return false;
+  }
+  return true;
+}
+
 }  // namespace c2a

@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <fstream>
 #include <mutex>
 #include <new>
 #include <string>
@@ -20,6 +21,8 @@
 #include "ccd_core.h"
 #include "ccd_kernel.h"
 #include "scorer.h"
+#include "static_kernel.h"
+#include "bvh_gpu.h"
 #include "tuning_table.h"
 
 namespace c2a {
@@ -90,6 +93,7 @@ struct c2a_mesh {
   Bvh host;
   DevBuf nodes, tris;
   int64_t ntri = 0;
+  uint64_t src_digest = 0;   // the triangles, ids and vertex indices it was made from
 };
 
 struct c2a_batch {
@@ -237,8 +241,12 @@ void params_to_args(KArgs& a, const c2a_params* p) {
 // c2a_amd/tuning.json by the autotuner): the template depth is the deepest
 // bucket's (5 for the learned order, its one instantiated depth), and the
 // replay arena is sized for the largest log among the buckets the batch's
-// queries use (an explicit replay_log >= 0 overrides every bucket's).
-void apply_tuning(c2a_batch* b, const std::vector<int32_t>& pairs, int replay_log) {
+// queries use (an explicit replay_log >= 0 or ext_levels >= 0 overrides every
+// bucket's).  The extension levels are tuned for the plain order only: under
+// the learned order, and when the batch logs training samples (which are
+// drawn from the replayed records), they stay at 0 unless given.
+void apply_tuning(c2a_batch* b, const std::vector<int32_t>& pairs, int replay_log,
+                  int ext_levels) {
   if (kTuneRows <= 0) return;
   const int rows = std::min(kTuneRows, C2A_TUNE_MAX);
   KArgs& t = b->tune;
@@ -249,6 +257,10 @@ void apply_tuning(c2a_batch* b, const std::vector<int32_t>& pairs, int replay_lo
     t.tune_depth[r] = std::min(6, std::max(1, kTune[r][1]));
     t.tune_miss[r] = std::max(0, kTune[r][2]);
     t.tune_log[r] = replay_log >= 0 ? replay_log : kTune[r][3];
+    t.tune_ext[r] = ext_levels >= 0 ? std::min(8, ext_levels)
+                                    : ((b->scorer || b->sample_cap > 0)
+                                           ? 0
+                                           : std::min(8, std::max(0, kTune[r][4])));
     kmax = std::max(kmax, t.tune_depth[r]);
   }
   b->K = b->scorer ? 5 : kmax;
@@ -281,6 +293,7 @@ int fill_args(c2a_batch* b, const c2a_params* p) {
     a.tune_depth[r] = b->tune.tune_depth[r];
     a.tune_miss[r] = b->tune.tune_miss[r];
     a.tune_log[r] = b->tune.tune_log[r];
+    a.tune_ext[r] = b->tune.tune_ext[r];
   }
   a.logcap = b->logcap;
   a.arena = (unsigned char*)b->d_arena.p;
@@ -454,18 +467,76 @@ void c2a_params_default(c2a_params* p) {
   p->sample_rate = 0.0;
   p->sample_cap = 0;
   p->miss_depth = 2;
-  p->ext_levels = 0;
+  p->ext_levels = -1;
   p->translation = 1;
   p->strict = 0;
 }
+
+namespace {
+
+// 64-bit digest of a byte range, eight bytes at a time (splitmix64 of the
+// running state xor each word; the tail is zero-padded).
+uint64_t digest_bytes(uint64_t h, const void* data, size_t n) {
+  const unsigned char* p = (const unsigned char*)data;
+  auto mix = [](uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+  };
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    memcpy(&w, p + i, 8);
+    h = mix(h ^ w);
+  }
+  if (i < n) {
+    uint64_t w = 0;
+    memcpy(&w, p + i, n - i);
+    h = mix(h ^ w);
+  }
+  return mix(h ^ (uint64_t)n);
+}
+
+uint64_t source_digest(const double* xyz9, const int32_t* ids, const int32_t* vidx3, int64_t n) {
+  uint64_t h = digest_bytes(0x43324142u, xyz9, (size_t)n * 9 * sizeof(double));
+  h = ids ? digest_bytes(h, ids, (size_t)n * 4) : digest_bytes(h, "noids", 5);
+  h = vidx3 ? digest_bytes(h, vidx3, (size_t)n * 12) : digest_bytes(h, "novidx", 6);
+  return h;
+}
+
+uint64_t payload_digest(const Bvh& b) {
+  uint64_t h = digest_bytes(0x50415931u, b.nodes.data(), b.nodes.size() * sizeof(GpuNode));
+  h = digest_bytes(h, b.tris.data(), b.tris.size() * sizeof(double));
+  h = digest_bytes(h, b.tri_id.data(), b.tri_id.size() * sizeof(int32_t));
+  return digest_bytes(h, b.tri_vidx.data(), b.tri_vidx.size() * sizeof(int32_t));
+}
+
+const char kCacheMagic[8] = {'C', '2', 'A', 'B', 'V', 'H', 0, 1};
+struct CacheHeader {
+  char magic[8];
+  uint32_t version, node_bytes;
+  int64_t ntri;
+  int32_t max_depth, pad;
+  uint64_t src_digest, payload_digest;
+};
+static_assert(sizeof(CacheHeader) == 48, "cache header layout");
+
+int check_mesh_input(const double* tri_xyz9, int64_t ntri) {
+  if (ntri <= 0) return fail(C2A_ERR_EMPTY_MODEL, "model has no triangles");
+  if (!tri_xyz9) return fail(C2A_ERR_BAD_ARG, "null triangle array");
+  if (ntri > (1 << 28)) return fail(C2A_ERR_BAD_ARG, "too many triangles");
+  return C2A_OK;
+}
+
+}  // namespace
 
 int c2a_mesh_create(const double* tri_xyz9, const int32_t* tri_ids, const int32_t* vert_idx3,
                     int64_t ntri, c2a_mesh** out) {
   if (!out) return fail(C2A_ERR_BAD_ARG, "null output handle");
   *out = nullptr;
-  if (ntri <= 0) return fail(C2A_ERR_EMPTY_MODEL, "model has no triangles");
-  if (!tri_xyz9) return fail(C2A_ERR_BAD_ARG, "null triangle array");
-  if (ntri > (1 << 28)) return fail(C2A_ERR_BAD_ARG, "too many triangles");
+  const int rc = check_mesh_input(tri_xyz9, ntri);
+  if (rc) return rc;
   c2a_mesh* m = new (std::nothrow) c2a_mesh();
   if (!m) return fail(C2A_ERR_OOM, "host allocation failed");
   try {
@@ -478,6 +549,115 @@ int c2a_mesh_create(const double* tri_xyz9, const int32_t* tri_ids, const int32_
     return fail(C2A_ERR_OOM, "host allocation failed during the build");
   }
   m->ntri = ntri;
+  m->src_digest = source_digest(tri_xyz9, tri_ids, vert_idx3, ntri);
+  *out = m;
+  return C2A_OK;
+}
+
+int c2a_mesh_create_device(const double* tri_xyz9, const int32_t* tri_ids,
+                           const int32_t* vert_idx3, int64_t ntri, void* stream,
+                           c2a_mesh** out) {
+  if (!out) return fail(C2A_ERR_BAD_ARG, "null output handle");
+  *out = nullptr;
+  int rc = check_mesh_input(tri_xyz9, ntri);
+  if (rc) return rc;
+  if ((rc = check_device())) return rc;
+  c2a_mesh* m = new (std::nothrow) c2a_mesh();
+  if (!m) return fail(C2A_ERR_OOM, "host allocation failed");
+  void *nodes = nullptr, *tris = nullptr;
+  hipError_t e;
+  try {
+    e = build_bvh_device(tri_xyz9, tri_ids, vert_idx3, ntri, &m->host, &nodes, &tris,
+                         (hipStream_t)stream);
+  } catch (const std::bad_alloc&) {
+    delete m;
+    return fail(C2A_ERR_OOM, "host allocation failed during the build");
+  }
+  if (e != hipSuccess) {
+    delete m;
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? fail(C2A_ERR_OOM, "device memory exhausted by the build")
+                                    : hip_fail(e, "device hierarchy build");
+  }
+  m->nodes.p = nodes;
+  m->nodes.bytes = m->host.nodes.size() * sizeof(GpuNode);
+  m->tris.p = tris;
+  m->tris.bytes = m->host.tris.size() * sizeof(double);
+  m->ntri = ntri;
+  m->src_digest = source_digest(tri_xyz9, tri_ids, vert_idx3, ntri);
+  *out = m;
+  return C2A_OK;
+}
+
+int c2a_mesh_save(const c2a_mesh* m, const char* path) {
+  if (!m || !path) return fail(C2A_ERR_BAD_ARG, "null argument");
+  CacheHeader h;
+  memset(&h, 0, sizeof(h));
+  memcpy(h.magic, kCacheMagic, 8);
+  h.version = 1;
+  h.node_bytes = (uint32_t)sizeof(GpuNode);
+  h.ntri = m->ntri;
+  h.max_depth = m->host.max_depth;
+  h.src_digest = m->src_digest;
+  h.payload_digest = payload_digest(m->host);
+  const std::string tmp = std::string(path) + ".tmp";
+  {
+    std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+    if (!f) return fail(C2A_ERR_BAD_ARG, "cannot write " + tmp);
+    f.write((const char*)&h, sizeof(h));
+    f.write((const char*)m->host.nodes.data(),
+            (std::streamsize)(m->host.nodes.size() * sizeof(GpuNode)));
+    f.write((const char*)m->host.tris.data(),
+            (std::streamsize)(m->host.tris.size() * sizeof(double)));
+    f.write((const char*)m->host.tri_id.data(), (std::streamsize)(m->host.tri_id.size() * 4));
+    f.write((const char*)m->host.tri_vidx.data(), (std::streamsize)(m->host.tri_vidx.size() * 4));
+    if (!f) return fail(C2A_ERR_BAD_ARG, "write failed: " + tmp);
+  }
+  // written under a temporary name, then moved: readers never see half a file
+  if (rename(tmp.c_str(), path) != 0)
+    return fail(C2A_ERR_BAD_ARG, std::string("cannot rename to ") + path);
+  return C2A_OK;
+}
+
+int c2a_mesh_load(const char* path, const double* tri_xyz9, const int32_t* tri_ids,
+                  const int32_t* vert_idx3, int64_t ntri, c2a_mesh** out) {
+  if (!out || !path) return fail(C2A_ERR_BAD_ARG, "null argument");
+  *out = nullptr;
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(C2A_ERR_CACHE, std::string("cannot open ") + path);
+  CacheHeader h;
+  if (!f.read((char*)&h, sizeof(h)) || memcmp(h.magic, kCacheMagic, 8) != 0 || h.version != 1 ||
+      h.node_bytes != sizeof(GpuNode) || h.ntri <= 0 || h.ntri > (1 << 28))
+    return fail(C2A_ERR_CACHE, std::string("not a hierarchy file: ") + path);
+  if (tri_xyz9 &&
+      (ntri != h.ntri || source_digest(tri_xyz9, tri_ids, vert_idx3, ntri) != h.src_digest))
+    return fail(C2A_ERR_CACHE, std::string("hierarchy file built from other triangles: ") + path);
+  c2a_mesh* m = new (std::nothrow) c2a_mesh();
+  if (!m) return fail(C2A_ERR_OOM, "host allocation failed");
+  const int64_t n = h.ntri;
+  try {
+    m->host.nodes.resize((size_t)(2 * n - 1));
+    m->host.tris.resize((size_t)n * 9);
+    m->host.tri_id.resize((size_t)n);
+    m->host.tri_vidx.resize((size_t)n * 3);
+  } catch (const std::bad_alloc&) {
+    delete m;
+    return fail(C2A_ERR_OOM, "host allocation failed");
+  }
+  f.read((char*)m->host.nodes.data(), (std::streamsize)(m->host.nodes.size() * sizeof(GpuNode)));
+  f.read((char*)m->host.tris.data(), (std::streamsize)(m->host.tris.size() * sizeof(double)));
+  f.read((char*)m->host.tri_id.data(), (std::streamsize)(m->host.tri_id.size() * 4));
+  f.read((char*)m->host.tri_vidx.data(), (std::streamsize)(m->host.tri_vidx.size() * 4));
+  bool ok = (bool)f;
+  char extra;
+  if (ok && f.read(&extra, 1)) ok = false;   // trailing bytes: not this format
+  if (!ok || payload_digest(m->host) != h.payload_digest) {
+    delete m;
+    return fail(C2A_ERR_CACHE, std::string("damaged hierarchy file: ") + path);
+  }
+  m->host.max_depth = h.max_depth;
+  m->ntri = n;
+  m->src_digest = h.src_digest;
   *out = m;
   return C2A_OK;
 }
@@ -575,7 +755,7 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     b->scorer = pp->order == C2A_ORDER_LEARNED ? pp->scorer : nullptr;
     b->sample_cap = pp->sample_cap;
     // spec_depth 0: the autotuned knobs of each query's size bucket
-    if (pp->spec_depth == 0) apply_tuning(b, pairs, pp->replay_log);
+    if (pp->spec_depth == 0) apply_tuning(b, pairs, pp->replay_log, pp->ext_levels);
   }
   b->blocks = grid_blocks(b->K, b->CAP, b->NW, nq, b->scorer != nullptr);
   const bool feats = b->scorer || b->sample_cap > 0;
@@ -984,6 +1164,158 @@ int c2a_collide_distance(c2a_mesh* const* meshes, int32_t nmesh, const double* p
                           stream, 1);
 }
 
+namespace {
+
+static_assert(sizeof(StaticOut) == sizeof(c2a_static_result), "static result layout");
+
+// The static queries of static_query.h: one thread per query over a
+// workspace of per-thread stacks / queues sized for the deepest pair of
+// hierarchies, with fewer threads when that would not fit in a quarter of
+// the free device memory.
+int static_queries(c2a_mesh* const* meshes, int32_t nmesh, int32_t kind, const double* poses,
+                   const int32_t* mesh_pairs, const int32_t* seed_tris, const double* params,
+                   double rel_err, double abs_err, int32_t qsize, int32_t flag, int64_t nq,
+                   int32_t pair_cap, int32_t* pairs_out, void* out, void* stream) {
+  if (nq < 0 || (nq > 0 && (!poses || !mesh_pairs || !out)))
+    return fail(C2A_ERR_BAD_ARG, "bad arrays");
+  if (nq > 0 && (kind == SQ_DISTANCE_CS || kind == SQ_TOLERANCE) && !params)
+    return fail(C2A_ERR_BAD_ARG, "null per-query bounds");
+  if (kind == SQ_COLLIDE && (pair_cap < 0 || (pair_cap > 0 && nq > 0 && !pairs_out)))
+    return fail(C2A_ERR_BAD_ARG, "bad pair list");
+  if (kind == SQ_COLLIDE && flag != 1 && flag != 2)
+    return fail(C2A_ERR_BAD_ARG, "flag must be 1 (all contacts) or 2 (first contact)");
+  if (qsize > (1 << 20)) return fail(C2A_ERR_BAD_ARG, "qsize above 2^20");
+  if (kind == SQ_DISTANCE_QUEUE && !(rel_err >= 0 && abs_err >= 0))
+    return fail(C2A_ERR_BAD_ARG, "error bounds must be >= 0");
+  if (nq == 0) return C2A_OK;
+  int rc = check_device();
+  if (rc) return rc;
+  std::vector<MeshDev> tab;
+  if ((rc = make_mesh_table(meshes, nmesh, tab))) return rc;
+  std::vector<StaticQueryDev> hq((size_t)nq);
+  int depth_sum = 0;
+  for (int64_t i = 0; i < nq; i++) {
+    StaticQueryDev& q = hq[(size_t)i];
+    memset(&q, 0, sizeof(q));
+    for (int k = 0; k < 2; k++) {
+      const int m = mesh_pairs[2 * i + k];
+      if (m < 0 || m >= nmesh) return fail(C2A_ERR_BAD_ARG, "mesh index out of range");
+      q.mesh[k] = m;
+      int t = seed_tris ? seed_tris[2 * i + k] : 0;
+      if (t < 0) t = 0;
+      if (t >= tab[(size_t)m].ntri) return fail(C2A_ERR_BAD_ARG, "seed triangle out of range");
+      q.seed[k] = t;
+    }
+    depth_sum = std::max(depth_sum, tab[(size_t)q.mesh[0]].depth + tab[(size_t)q.mesh[1]].depth);
+    unpack_pose(poses + 24 * i, q.R1, q.T1);
+    unpack_pose(poses + 24 * i + 12, q.R2, q.T2);
+    q.param = params ? params[i] : 0.0;
+  }
+  const int64_t per_thread = sq_ws_entries(kind, qsize, depth_sum);
+  const size_t thread_bytes = (size_t)per_thread * sizeof(SqEntry);
+  size_t free_b = 0, total_b = 0;
+  size_t budget = (size_t)1 << 30;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) budget = free_b / 4;
+  int64_t blocks = (nq + kStaticBlock - 1) / kStaticBlock;
+  blocks = std::min<int64_t>(blocks, 4096);
+  while (blocks > 1 && (size_t)blocks * kStaticBlock * thread_bytes > budget) blocks /= 2;
+  if ((size_t)blocks * kStaticBlock * thread_bytes > budget)
+    return fail(C2A_ERR_OOM, "search workspace for one workgroup exceeds the device budget");
+  const size_t out_bytes = kind == SQ_DISTANCE_QUEUE ? sizeof(StaticOut) : sizeof(c2a_static_result);
+  DevBuf dm, dq, dout, dc, dws, dp;
+  if ((rc = dm.alloc(tab.size() * sizeof(MeshDev))) ||
+      (rc = dq.alloc((size_t)nq * sizeof(StaticQueryDev))) ||
+      (rc = dout.alloc((size_t)nq * out_bytes)) || (rc = dc.alloc(sizeof(int64_t))) ||
+      (rc = dws.alloc((size_t)blocks * kStaticBlock * thread_bytes)))
+    return rc;
+  const size_t pair_bytes = kind == SQ_COLLIDE ? (size_t)nq * pair_cap * 2 * sizeof(int32_t) : 0;
+  if (pair_bytes && (rc = dp.alloc(pair_bytes))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpy(dm.p, tab.data(), tab.size() * sizeof(MeshDev), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dq.p, hq.data(), (size_t)nq * sizeof(StaticQueryDev), hipMemcpyHostToDevice));
+  StaticArgs a;
+  memset(&a, 0, sizeof(a));
+  a.meshes = (const MeshDev*)dm.p;
+  a.q = (const StaticQueryDev*)dq.p;
+  a.out = (StaticOut*)dout.p;
+  a.pairs = pair_bytes ? (int32_t*)dp.p : nullptr;
+  a.ws = (SqEntry*)dws.p;
+  a.counter = (int64_t*)dc.p;
+  a.nq = nq;
+  a.ws_entries = per_thread;
+  a.kind = kind;
+  a.qsize = qsize;
+  a.flag = flag;
+  a.pair_cap = kind == SQ_COLLIDE ? pair_cap : 0;
+  a.rel_err = rel_err;
+  a.abs_err = abs_err;
+  HIP_TRY(launch_static(a, (int)blocks, s));
+  if (kind == SQ_DISTANCE_QUEUE) {
+    // the distance-query record: StaticOut's tri is the models' last_tri
+    std::vector<StaticOut> h((size_t)nq);
+    HIP_TRY(hipMemcpyAsync(h.data(), dout.p, (size_t)nq * sizeof(StaticOut),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    c2a_distance_result* r = (c2a_distance_result*)out;
+    for (int64_t i = 0; i < nq; i++) {
+      const StaticOut& o = h[(size_t)i];
+      memset(&r[i], 0, sizeof(r[i]));
+      r[i].distance = o.distance;
+      for (int k = 0; k < 3; k++) {
+        r[i].p1[k] = o.p1[k];
+        r[i].p2[k] = o.p2[k];
+      }
+      r[i].bv_tests = o.bv_tests;
+      r[i].tri_tests = o.tri_tests;
+      r[i].last_tri[0] = o.tri[0];
+      r[i].last_tri[1] = o.tri[1];
+      r[i].flags = o.flags;
+    }
+    return C2A_OK;
+  }
+  HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)nq * sizeof(c2a_static_result),
+                         hipMemcpyDeviceToHost, s));
+  if (pair_bytes)
+    HIP_TRY(hipMemcpyAsync(pairs_out, dp.p, pair_bytes, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return C2A_OK;
+}
+
+}  // namespace
+
+int c2a_distance_queue(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                       const int32_t* mesh_pairs, const int32_t* seed_tris, double rel_err,
+                       double abs_err, int32_t qsize, int64_t nq, c2a_distance_result* out,
+                       void* stream) {
+  if (qsize <= 2)
+    return c2a_distance(meshes, nmesh, poses, mesh_pairs, seed_tris, rel_err, abs_err, nq, out,
+                        stream);
+  return static_queries(meshes, nmesh, SQ_DISTANCE_QUEUE, poses, mesh_pairs, seed_tris, nullptr,
+                        rel_err, abs_err, qsize, 0, nq, 0, nullptr, out, stream);
+}
+
+int c2a_distance_contact_space(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                               const int32_t* mesh_pairs, const int32_t* seed_tris,
+                               const double* upper_bounds, int64_t nq, c2a_static_result* out,
+                               void* stream) {
+  return static_queries(meshes, nmesh, SQ_DISTANCE_CS, poses, mesh_pairs, seed_tris,
+                        upper_bounds, 0.0, 0.0, 2, 0, nq, 0, nullptr, out, stream);
+}
+
+int c2a_tolerance(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                  const int32_t* mesh_pairs, const double* tolerances, int32_t qsize,
+                  int64_t nq, c2a_static_result* out, void* stream) {
+  return static_queries(meshes, nmesh, SQ_TOLERANCE, poses, mesh_pairs, nullptr, tolerances, 0.0,
+                        0.0, qsize, 0, nq, 0, nullptr, out, stream);
+}
+
+int c2a_collide_pairs(c2a_mesh* const* meshes, int32_t nmesh, const double* poses,
+                      const int32_t* mesh_pairs, int32_t flag, int64_t nq, int32_t pair_cap,
+                      int32_t* pairs_out, c2a_static_result* out, void* stream) {
+  return static_queries(meshes, nmesh, SQ_COLLIDE, poses, mesh_pairs, nullptr, nullptr, 0.0, 0.0,
+                        2, flag, nq, pair_cap, pairs_out, out, stream);
+}
+
 int c2a_motion_prepare(const double* pose_start, const double* pose_end, double* motion24) {
   if (!pose_start || !pose_end || !motion24) return fail(C2A_ERR_BAD_ARG, "null argument");
   double R0[9], T0[3], R1[9], T1[3];
@@ -1013,6 +1345,7 @@ const char* c2a_status_string(int status) {
     case C2A_ERR_NO_DEVICE: return "no GPU device";
     case C2A_ERR_UNSUPPORTED: return "unsupported input";
     case C2A_ERR_EMPTY_MODEL: return "empty model";
+    case C2A_ERR_CACHE: return "hierarchy file not usable";
     default: return "unknown status";
   }
 }

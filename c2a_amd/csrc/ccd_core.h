@@ -38,6 +38,22 @@
 
 namespace c2a {
 
+#if !defined(__HIP_DEVICE_COMPILE__) && defined(C2A_EMU_SIMT)
+// Host-side test harness with 64 lanes (tests/emu/ccd_emu64.cpp): each lane
+// runs on its own fiber and the wavefront primitives below are barriers
+// that exchange the lanes' values.
+namespace simt {
+void sync();
+uint64_t ballot(bool p);
+int bcast(int v, int src);
+double bcastd(double v, int src);
+int excl_prefix_max(int v);
+double mind(double v);
+int maxi(int v);
+long long lane0(long long v);
+}  // namespace simt
+#endif
+
 C2A_HD int clz32(unsigned x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __clz((int)x);
@@ -56,6 +72,8 @@ C2A_HD void lane_sync() {
   __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#elif defined(C2A_EMU_SIMT)
+  simt::sync();
 #endif
 }
 
@@ -64,6 +82,8 @@ C2A_HD void lane_sync() {
 C2A_HD void mem_sync() {
 #if defined(__HIP_DEVICE_COMPILE__)
   __syncthreads();
+#elif defined(C2A_EMU_SIMT)
+  simt::sync();
 #endif
 }
 
@@ -284,6 +304,7 @@ struct Leader {
   int need_score;                // replayed records wait for the scorer
   int kdepth, kmiss, klog;       // this query's speculative depth, depth below
                                  // a miss and replay log capacity
+  int kext;                      // levels of speculative extension in the replay
   unsigned char* ab;             // this wavefront's replay arena
 };
 
@@ -306,16 +327,12 @@ C2A_HD void slot_to_entry(Entry& dst, const Slot& src, int sg, int flags, int nn
 }
 
 // Leaf-pair values: C2A.cpp:1145-1166 up to the distance-dependent update.
-C2A_HD void leaf_values(Slot& s, const MeshDev& M1, const MeshDev& M2, const double* Rrel,
-                        const double* Trel, const double* r1, const double* tt1,
-                        const Motion& mo1, const Motion& mo2, int b1, int b2, bool contact) {
-  node_ptr A1 = nodes_of(M1) + b1;
-  node_ptr A2 = nodes_of(M2) + b2;
-  int t1 = -A1->first_child - 1, t2 = -A2->first_child - 1;
-  double p[3], q[3], ta[9], tb[9];
-  gdouble_ptr g1 = tris_of(M1) + 9 * (int64_t)t1;
-  gdouble_ptr g2 = tris_of(M2) + 9 * (int64_t)t2;
-  for (int i = 0; i < 9; i++) { ta[i] = g1[i]; tb[i] = g2[i]; }
+// Leaf-pair values from the two triangles (9 doubles each, already loaded).
+C2A_HD void leaf_values_tris(Slot& s, node_ptr A1, node_ptr A2, const double* ta,
+                             const double* tb, int t1, int t2, const double* Rrel,
+                             const double* Trel, const double* r1, const double* tt1,
+                             const Motion& mo1, const Motion& mo2, bool contact) {
+  double p[3], q[3];
   int feat[4];
   double dTri = tri_distance(Rrel, Trel, ta, tb, p, q, feat);
   double mleaf = 0.0;
@@ -338,6 +355,19 @@ C2A_HD void leaf_values(Slot& s, const MeshDev& M1, const MeshDev& M2, const dou
   s.t1 = t1;
   s.t2 = t2;
   s.flags = (s.flags & 3) | ((feat[0] != -1 && feat[2] != -1) ? 4 : 0);
+}
+
+C2A_HD void leaf_values(Slot& s, const MeshDev& M1, const MeshDev& M2, const double* Rrel,
+                        const double* Trel, const double* r1, const double* tt1,
+                        const Motion& mo1, const Motion& mo2, int b1, int b2, bool contact) {
+  node_ptr A1 = nodes_of(M1) + b1;
+  node_ptr A2 = nodes_of(M2) + b2;
+  int t1 = -A1->first_child - 1, t2 = -A2->first_child - 1;
+  double ta[9], tb[9];
+  gdouble_ptr g1 = tris_of(M1) + 9 * (int64_t)t1;
+  gdouble_ptr g2 = tris_of(M2) + 9 * (int64_t)t2;
+  for (int i = 0; i < 9; i++) { ta[i] = g1[i]; tb[i] = g2[i]; }
+  leaf_values_tris(s, A1, A2, ta, tb, t1, t2, Rrel, Trel, r1, tt1, mo1, mo2, contact);
 }
 
 // One level of descent (C2A.cpp:1195-1226): replaces (b1, b2) and (R, T) by
@@ -365,6 +395,55 @@ C2A_HD bool descend(node_ptr N1, node_ptr N2, int& b1, int& b2, int bit, double*
     node_ptr cn = N2 + c;
     for (int i = 0; i < 9; i++) CR[i] = cn->R[i];
     for (int i = 0; i < 3; i++) CT[i] = cn->Tr[i];
+    mxm(Rn, R, CR);
+    mxvpv(Tn, R, CT, T);
+    b2 = c;
+  }
+  for (int i = 0; i < 9; i++) R[i] = Rn[i];
+  for (int i = 0; i < 3; i++) T[i] = Tn[i];
+  return true;
+}
+
+// First-child link and descend-rule size of a node (the fields of its
+// second line that a descent reads), carried from the level that fetched the
+// node, so that a speculative chain pays one dependent fetch per level: the
+// child's transform line and its head are loaded together.
+struct NodeHead {
+  int32_t fc;
+  double size;
+};
+
+C2A_HD NodeHead head_of(node_ptr n) {
+  NodeHead h;
+  h.fc = n->first_child;
+  h.size = n->size;
+  return h;
+}
+
+// descend() with the heads of (b1, b2) given in h1/h2 and updated to the
+// heads of the child pair.
+C2A_HD bool descend_h(node_ptr N1, node_ptr N2, int& b1, int& b2, NodeHead& h1, NodeHead& h2,
+                      int bit, double* R, double* T) {
+  C2A_CHECK(b1 >= 0 && b2 >= 0);
+  const bool l1 = h1.fc < 0, l2 = h2.fc < 0;
+  if (l1 && l2) return false;
+  double Rn[9], Tn[3], Tt[3], CR[9], CT[3];
+  if (l2 || (!l1 && (h1.size > h2.size))) {
+    const int c = h1.fc + bit;
+    node_ptr cn = N1 + c;
+    for (int i = 0; i < 9; i++) CR[i] = cn->R[i];
+    for (int i = 0; i < 3; i++) CT[i] = cn->Tr[i];
+    h1 = head_of(cn);
+    mtxm(Rn, CR, R);
+    sub3(Tt, T, CT);
+    mtxv(Tn, CR, Tt);
+    b1 = c;
+  } else {
+    const int c = h2.fc + bit;
+    node_ptr cn = N2 + c;
+    for (int i = 0; i < 9; i++) CR[i] = cn->R[i];
+    for (int i = 0; i < 3; i++) CT[i] = cn->Tr[i];
+    h2 = head_of(cn);
     mxm(Rn, R, CR);
     mxvpv(Tn, R, CT, T);
     b2 = c;
@@ -530,15 +609,26 @@ C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const M
   for (int i = 0; i < 9; i++) R[i] = g.X.R[i];
   for (int i = 0; i < 3; i++) T[i] = g.X.T[i];
   int b1 = g.X.b1, b2 = g.X.b2;
+  NodeHead h1 = head_of(N1 + b1), h2 = head_of(N2 + b2);
   for (int lv = 0; lv < lvl; lv++) {
     const int bit = ((h + 1) >> (lvl - 1 - lv)) & 1;
-    if (!descend(N1, N2, b1, b2, bit, R, T)) {
+    if (!descend_h(N1, N2, b1, b2, h1, h2, bit, R, T)) {
       s.valid = 0;
       return;
     }
   }
   double d, mt;
   C2A_CHECK(b1 >= 0 && b2 >= 0 && b1 < 2 * M1.ntri - 1 && b2 < 2 * M2.ntri - 1);
+  // a leaf pair's triangles are fetched before its RSS test, so the fetch
+  // overlaps that arithmetic
+  const bool leafp = h1.fc < 0 && h2.fc < 0;
+  const int t1 = -h1.fc - 1, t2 = -h2.fc - 1;
+  double ta[9], tb[9];
+  if (leafp) {
+    gdouble_ptr g1 = tris_of(M1) + 9 * (int64_t)t1;
+    gdouble_ptr g2 = tris_of(M2) + 9 * (int64_t)t2;
+    for (int i = 0; i < 9; i++) { ta[i] = g1[i]; tb[i] = g2[i]; }
+  }
   const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], contact, d, mt);
   for (int i = 0; i < 9; i++) s.R[i] = R[i];
   for (int i = 0; i < 3; i++) s.T[i] = T[i];
@@ -548,8 +638,9 @@ C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const M
   s.b2 = b2;
   s.flags = fl;
   s.valid = 1;
-  if ((s.flags & 3) == 3)
-    leaf_values(s, M1, M2, g.Rrel, g.Trel, g.r1, g.tt1, g.mo[0], g.mo[1], b1, b2, contact);
+  if (leafp)
+    leaf_values_tris(s, N1 + b1, N2 + b2, ta, tb, t1, t2, g.Rrel, g.Trel, g.r1, g.tt1, g.mo[0],
+                     g.mo[1], contact);
 }
 
 template <int K, int CAP>
@@ -749,6 +840,8 @@ inline double root_rss_distance(const GpuNode* r1n, const GpuNode* r2n, const do
 C2A_HD uint64_t wave_ballot(bool p) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return (uint64_t)__ballot(p);
+#elif defined(C2A_EMU_SIMT)
+  return simt::ballot(p);
 #else
   return p ? 1u : 0u;
 #endif
@@ -765,6 +858,8 @@ C2A_HD int lanes_below(uint64_t m, int lane) {
 C2A_HD int lane_bcast(int v, int src) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __shfl(v, src);
+#elif defined(C2A_EMU_SIMT)
+  return simt::bcast(v, src);
 #else
   (void)src;
   return v;
@@ -773,6 +868,8 @@ C2A_HD int lane_bcast(int v, int src) {
 C2A_HD double lane_bcastd(double v, int src) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __shfl(v, src);
+#elif defined(C2A_EMU_SIMT)
+  return simt::bcastd(v, src);
 #else
   (void)src;
   return v;
@@ -787,6 +884,9 @@ C2A_HD int excl_prefix_max(int v, int lane) {
   }
   const int x = __shfl_up(v, 1u);
   return lane == 0 ? 0 : x;
+#elif defined(C2A_EMU_SIMT)
+  (void)lane;
+  return simt::excl_prefix_max(v);
 #else
   (void)v;
   (void)lane;
@@ -799,6 +899,8 @@ C2A_HD double wave_mind(double v) {
     const double u = __shfl_xor(v, o);
     v = u < v ? u : v;
   }
+#elif defined(C2A_EMU_SIMT)
+  v = simt::mind(v);
 #endif
   return v;
 }
@@ -808,6 +910,8 @@ C2A_HD int wave_maxi(int v) {
     const int u = __shfl_xor(v, o);
     v = u > v ? u : v;
   }
+#elif defined(C2A_EMU_SIMT)
+  v = simt::maxi(v);
 #endif
   return v;
 }
@@ -833,6 +937,8 @@ C2A_HD long long lane0_bcast64(long long v) {
   const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u);
   const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
   return (long long)(((unsigned long long)hi << 32) | lo);
+#elif defined(C2A_EMU_SIMT)
+  return simt::lane0(v);
 #else
   return v;
 #endif
@@ -894,7 +1000,8 @@ C2A_HD void replay_record(Wave<K, CAP>& g, const Leader& L, const KArgs& a, cons
   for (int i = 0; i < 9; i++) R[i] = pxf[i];
   for (int i = 0; i < 3; i++) T[i] = pxf[9 + i];
   int b1 = le.b1, b2 = le.b2;
-  descend(N1, N2, b1, b2, side, R, T);   // an expanded pair is never a leaf pair
+  NodeHead h1 = head_of(N1 + b1), h2 = head_of(N2 + b2);
+  descend_h(N1, N2, b1, b2, h1, h2, side, R, T);   // an expanded pair is never a leaf pair
   double d, mt, mb[2] = {0.0, 0.0};
   const int fl = pair_values(N1, N2, b1, b2, R, T, g.r1, g.mo[0], g.mo[1], false, d, mt, mb);
   if (a.feats) {
@@ -991,10 +1098,10 @@ C2A_HD void replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl)
   // replayed instead of walking below them one round at a time.  Values never
   // depend on the bound, so this changes no result.
   int ntot = n;
-  if (a.ext_levels > 0) {
+  if (L.kext > 0) {
     LogEnt C2A_GLOBAL* lgw = const_cast<LogEnt C2A_GLOBAL*>(lg);
     int lo = 1, hi = nrec;
-    for (int pass = 0; pass < a.ext_levels; pass++) {
+    for (int pass = 0; pass < L.kext; pass++) {
       if (lane == 0) g.next = 0;
       lane_sync();
       const int room = L.klog - ntot;
@@ -1206,7 +1313,7 @@ C2A_HD int scan(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
       E.info = 3;
     }
     const int lev = E.info >> 8;
-    C2A_CHECK(lev >= 1 && lev < CAP + 2 && E.size >= 1);
+    C2A_CHECK(!valid || (lev >= 1 && lev < CAP + 2 && E.size >= 1));
     const bool internal = (E.info & 3) != 3;
     const bool hasch = (E.info & 4) != 0;
     const bool desc =
@@ -1767,6 +1874,7 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   L.kdepth = K;
   L.kmiss = a.miss_depth;
   L.klog = a.logcap;
+  L.kext = a.ext_levels;
   L.flags = 0;
   L.bv = 0;
   L.tri = 0;
@@ -1915,6 +2023,7 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
     L.kdepth = a.tune_depth[b] < K ? a.tune_depth[b] : K;
     L.kmiss = a.tune_miss[b];
     L.klog = a.tune_log[b] < a.logcap ? a.tune_log[b] : a.logcap;
+    L.kext = a.tune_ext[b];
   }
   L.mode = MODE_QUERY;
   L.nlog_prev = 0;

@@ -198,13 +198,15 @@ struct KArgs {
                            // pair bounds mint (motions carry the world axis)
   // Per-size-bucket traversal knobs (autotuned): a CA query takes the first
   // bucket whose triangle bound covers its larger mesh and uses that
-  // bucket's speculative depth (<= K), depth below a miss and replay log
-  // (<= logcap).  ntune == 0: every query uses K, miss_depth and logcap.
+  // bucket's speculative depth (<= K), depth below a miss, replay log
+  // (<= logcap) and replay extension levels.  ntune == 0: every query uses
+  // K, miss_depth, logcap and ext_levels.
   int32_t ntune;
   int32_t tune_ntri[C2A_TUNE_MAX];
   int32_t tune_depth[C2A_TUNE_MAX];
   int32_t tune_miss[C2A_TUNE_MAX];
   int32_t tune_log[C2A_TUNE_MAX];
+  int32_t tune_ext[C2A_TUNE_MAX];
   ScorerDev scorer;        // learned child order when scorer.on
   // training samples of replayed pairs: a pair is logged when the top 32
   // bits of mix64(key) are below sample_thr (2^32 = every pair)

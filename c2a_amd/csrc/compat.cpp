@@ -424,11 +424,12 @@ int distance_query(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C
   const int32_t pairs[2] = {0, 1};
   const int32_t seeds[2] = {o1->tri_index(o1->last_tri), o2->tri_index(o2->last_tri)};
   c2a_distance_result r;
+  // C2A_Collide's distance form ignores qsize (C2A_PQP.cpp:1200-1279)
   const int rc = overlap_only
                      ? c2a_collide_distance(meshes, 2, poses, pairs, seeds, rel_err, abs_err, 1,
                                             &r, nullptr)
-                     : c2a_distance(meshes, 2, poses, pairs, seeds, rel_err, abs_err, 1, &r,
-                                    nullptr);
+                     : c2a_distance_queue(meshes, 2, poses, pairs, seeds, rel_err, abs_err,
+                                          qsize, 1, &r, nullptr);
   if (rc != C2A_OK) return -1;
   // [R, T] = [R1, T1]^T [R2, T2] (C2A_PQP.cpp:990-993)
   for (int i = 0; i < 3; i++) {
@@ -444,7 +445,7 @@ int distance_query(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C
     res->p1[i] = r.p1[i];
     res->p2[i] = r.p2[i];
   }
-  res->qsize = qsize;
+  if (!overlap_only && qsize > 2) res->qsize = qsize;   // C2A_PQP.cpp:1034
   res->num_bv_tests = (int)r.bv_tests;
   res->num_tri_tests = (int)r.tri_tests;
   o1->last_tri = &o1->tris[r.last_tri[0]];
@@ -456,7 +457,128 @@ int distance_query(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C
   return PQP_OK;
 }
 
+// [R, T] = [R1, T1]^T [R2, T2]: model 2 in model 1's frame (C2A_PQP.cpp:990-993)
+void relative_pose(PQP_REAL R1[3][3], PQP_REAL T1[3], PQP_REAL R2[3][3], PQP_REAL T2[3],
+                   PQP_REAL R[3][3], PQP_REAL T[3]) {
+  for (int i = 0; i < 3; i++) {
+    for (int j = 0; j < 3; j++)
+      R[i][j] = R1[0][i] * R2[0][j] + R1[1][i] * R2[1][j] + R1[2][i] * R2[2][j];
+    T[i] = R1[0][i] * (T2[0] - T1[0]) + R1[1][i] * (T2[1] - T1[1]) +
+           R1[2][i] * (T2[2] - T1[2]);
+  }
+}
+
+// p2 <- R^T (p2 - T): the reference's closing step of a static query, run
+// on a point it did not compute too (C2A_PQP.cpp:1484-1488, 1834-1838).
+void p2_to_frame2(PQP_REAL R[3][3], PQP_REAL T[3], PQP_REAL p2[3]) {
+  PQP_REAL u[3] = {p2[0] - T[0], p2[1] - T[1], p2[2] - T[2]};
+  for (int i = 0; i < 3; i++) p2[i] = R[0][i] * u[0] + R[1][i] * u[1] + R[2][i] * u[2];
+}
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 }  // namespace
+
+int C2A_Distance_At_ContactSpace(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3],
+                                 C2A_Model* o1, PQP_REAL R2[3][3], PQP_REAL T2[3],
+                                 C2A_Model* o2, PQP_REAL upper_bound) {
+  if (o1->build_state != 2 || o2->build_state != 2) return PQP_ERR_UNPROCESSED_MODEL;
+  const auto t0 = std::chrono::steady_clock::now();
+  c2a_mesh* meshes[2] = {o1->engine_mesh(), o2->engine_mesh()};
+  double poses[24];
+  pose12_from(R1, T1, poses);
+  pose12_from(R2, T2, poses + 12);
+  const int32_t pairs[2] = {0, 1};
+  const int32_t seeds[2] = {o1->tri_index(o1->last_tri), o2->tri_index(o2->last_tri)};
+  c2a_static_result r;
+  if (c2a_distance_contact_space(meshes, 2, poses, pairs, seeds, &upper_bound, 1, &r, nullptr) !=
+      C2A_OK)
+    return -1;
+  relative_pose(R1, T1, R2, T2, res->R, res->T);
+  res->distance = r.distance;
+  res->abs_err = 0.0;
+  res->rel_err = 0.0;
+  res->num_bv_tests = (int)r.bv_tests;
+  res->num_tri_tests = (int)r.tri_tests;
+  if (r.found) {   // C2A_PQP.cpp:1311-1322
+    for (int i = 0; i < 3; i++) {
+      res->p1[i] = r.p1[i];
+      res->p2[i] = r.p2[i];
+    }
+    o1->last_tri = &o1->tris[r.tri[0]];
+    o2->last_tri = &o2->tris[r.tri[1]];
+    res->t1 = o1->last_tri->id;
+    res->t2 = o2->last_tri->id;
+  } else {
+    p2_to_frame2(res->R, res->T, res->p2);
+  }
+  res->query_time_secs = seconds_since(t0);
+  return PQP_OK;
+}
+
+int C2A_Collide(PQP_CollideResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
+                PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, int flag) {
+  if (o1->build_state != 2 || o2->build_state != 2) return PQP_ERR_UNPROCESSED_MODEL;
+  const auto t0 = std::chrono::steady_clock::now();
+  c2a_mesh* meshes[2] = {o1->engine_mesh(), o2->engine_mesh()};
+  double poses[24];
+  pose12_from(R1, T1, poses);
+  pose12_from(R2, T2, poses + 12);
+  const int32_t mp[2] = {0, 1};
+  const int32_t f = flag == C2A_FIRST_CONTACT ? 2 : 1;   // C2A_PQP.cpp:878, 898
+  c2a_static_result r;
+  std::vector<int32_t> list(2 * 256);
+  if (c2a_collide_pairs(meshes, 2, poses, mp, f, 1, 256, list.data(), &r, nullptr) != C2A_OK)
+    return -1;
+  if (r.found > 256) {   // the list was cut: run again with room for every pair
+    list.assign(2 * (size_t)r.found, 0);
+    if (c2a_collide_pairs(meshes, 2, poses, mp, f, 1, r.found, list.data(), &r, nullptr) !=
+        C2A_OK)
+      return -1;
+  }
+  relative_pose(R1, T1, R2, T2, res->R, res->T);
+  res->num_bv_tests = (int)r.bv_tests;
+  res->num_tri_tests = (int)r.tri_tests;
+  res->num_pairs = 0;   // keeps the list's memory (C2A_PQP.cpp:930-932)
+  for (int k = 0; k < r.found; k++)
+    res->Add(o1->tris[list[2 * k]].id, o2->tris[list[2 * k + 1]].id);
+  res->query_time_secs = seconds_since(t0);
+  return PQP_OK;
+}
+
+int C2A_Tolerance(PQP_ToleranceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
+                  PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL tolerance,
+                  int qsize) {
+  if (o1->build_state != 2 || o2->build_state != 2) return PQP_ERR_UNPROCESSED_MODEL;
+  const auto t0 = std::chrono::steady_clock::now();
+  c2a_mesh* meshes[2] = {o1->engine_mesh(), o2->engine_mesh()};
+  double poses[24];
+  pose12_from(R1, T1, poses);
+  pose12_from(R2, T2, poses + 12);
+  const int32_t mp[2] = {0, 1};
+  c2a_static_result r;
+  if (c2a_tolerance(meshes, 2, poses, mp, &tolerance, qsize, 1, &r, nullptr) != C2A_OK)
+    return -1;
+  relative_pose(R1, T1, R2, T2, res->R, res->T);
+  res->tolerance = tolerance < 0.0 ? 0.0 : tolerance;   // C2A_PQP.cpp:1786-1787
+  res->num_bv_tests = (int)r.bv_tests;
+  res->num_tri_tests = (int)r.tri_tests;
+  res->closer_than_tolerance = r.found;
+  if (qsize > 2) res->qsize = qsize;
+  if (r.found) {
+    res->distance = r.distance;
+    for (int i = 0; i < 3; i++) {
+      res->p1[i] = r.p1[i];
+      res->p2[i] = r.p2[i];
+    }
+  } else {
+    p2_to_frame2(res->R, res->T, res->p2);
+  }
+  res->query_time_secs = seconds_since(t0);
+  return PQP_OK;
+}
 
 int C2A_Distance(C2A_DistanceResult* res, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
                  PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,

@@ -134,10 +134,9 @@ struct C2A_DistanceResult {
 // Closest triangle pair of two models at the given poses, within the
 // relative and absolute error bounds (C2A.h:255-259, C2A_PQP.cpp:970-1057).
 // The first bound is the models' last_tri pair; both are updated.  qsize > 2
-// (the reference's priority-queue search) runs the same depth-first search,
-// which meets the same error bounds; its counters then differ from the
-// reference's.  Returns PQP_OK, PQP_ERR_UNPROCESSED_MODEL, or -1 when the
-// engine reports an error.
+// runs the best-first search over a queue of qsize tests
+// (C2ADistanceQueueRecurse, C2A_PQP.cpp:624-786).  Returns PQP_OK,
+// PQP_ERR_UNPROCESSED_MODEL, or -1 when the engine reports an error.
 int C2A_Distance(C2A_DistanceResult* result, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
                  PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,
                  PQP_REAL abs_err, int qsize = 2);
@@ -148,6 +147,109 @@ int C2A_Distance(C2A_DistanceResult* result, PQP_REAL R1[3][3], PQP_REAL T1[3], 
 int C2A_Collide(C2A_DistanceResult* result, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
                 PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL rel_err,
                 PQP_REAL abs_err, int qsize = 2);
+
+// C2A_Distance from distance = upper_bound, zero error bounds and no seed
+// pair, stopping at the first triangle pair closer than the bound
+// (C2A.h:241-244, C2A_PQP.cpp:1282-1496).  When one is found, distance, p1,
+// p2, t1, t2 and the models' last_tri describe it; otherwise distance is the
+// bound and, as in the reference, only p2 is mapped into model 2's frame.
+int C2A_Distance_At_ContactSpace(C2A_DistanceResult* result, PQP_REAL R1[3][3],
+                                 PQP_REAL T1[3], C2A_Model* o1, PQP_REAL R2[3][3],
+                                 PQP_REAL T2[3], C2A_Model* o2, PQP_REAL upper_bound);
+
+const int C2A_ALL_CONTACTS = 1;   // find all pairwise intersecting triangles
+const int C2A_FIRST_CONTACT = 2;  // report the first intersecting pair found
+
+// PQP's collision result (PQP 1.3 PQP_CollideResult; the reference passes it
+// to C2A_Collide, C2A.h:250-253): the intersecting pairs as caller ids.
+struct CollisionPair {
+  int id1;
+  int id2;
+};
+struct PQP_CollideResult {
+  int num_bv_tests;
+  int num_tri_tests;
+  double query_time_secs;
+  PQP_REAL R[3][3];
+  PQP_REAL T[3];
+  int num_pairs_alloced;
+  int num_pairs;
+  CollisionPair* pairs;
+
+  PQP_CollideResult() : num_bv_tests(0), num_tri_tests(0), query_time_secs(0.0),
+                        num_pairs_alloced(0), num_pairs(0), pairs(nullptr) {}
+  ~PQP_CollideResult() { delete[] pairs; }
+  PQP_CollideResult(const PQP_CollideResult&) = delete;
+  PQP_CollideResult& operator=(const PQP_CollideResult&) = delete;
+  void FreePairsList() {
+    num_pairs = num_pairs_alloced = 0;
+    delete[] pairs;
+    pairs = nullptr;
+  }
+  void SizeTo(int n) {   // grows or shrinks the list, keeping the pairs
+    if (n < num_pairs) return;
+    CollisionPair* p = new CollisionPair[n > 0 ? n : 1];
+    for (int i = 0; i < num_pairs; i++) p[i] = pairs[i];
+    delete[] pairs;
+    pairs = p;
+    num_pairs_alloced = n;
+  }
+  void Add(int a, int b) {
+    if (num_pairs >= num_pairs_alloced) SizeTo(num_pairs_alloced * 2 + 8);
+    pairs[num_pairs].id1 = a;
+    pairs[num_pairs].id2 = b;
+    num_pairs++;
+  }
+  int NumBVTests() { return num_bv_tests; }
+  int NumTriTests() { return num_tri_tests; }
+  double QueryTimeSecs() { return query_time_secs; }
+  int Colliding() { return num_pairs > 0; }
+  int NumPairs() { return num_pairs; }
+  int Id1(int k) { return pairs[k].id1; }
+  int Id2(int k) { return pairs[k].id2; }
+};
+
+// Every intersecting triangle pair under pairs of overlapping RSS volumes, in
+// visiting order; C2A_FIRST_CONTACT stops at the first (C2A.h:250-253,
+// C2A_PQP.cpp:798-968).  Returns PQP_OK, PQP_ERR_UNPROCESSED_MODEL, or -1
+// when the engine reports an error.
+int C2A_Collide(PQP_CollideResult* result, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
+                PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, int flag = C2A_ALL_CONTACTS);
+
+// PQP's tolerance result (PQP 1.3 PQP_ToleranceResult, filled by
+// C2A_Tolerance, C2A_PQP.cpp:1762-1845).
+struct PQP_ToleranceResult {
+  int num_bv_tests;
+  int num_tri_tests;
+  double query_time_secs;
+  PQP_REAL R[3][3];
+  PQP_REAL T[3];
+  int closer_than_tolerance;
+  int qsize;
+  PQP_REAL tolerance;
+  PQP_REAL distance;
+  PQP_REAL p1[3];
+  PQP_REAL p2[3];
+
+  int NumBVTests() { return num_bv_tests; }
+  int NumTriTests() { return num_tri_tests; }
+  double QueryTimeSecs() { return query_time_secs; }
+  int CloserThanTolerance() { return closer_than_tolerance; }
+  PQP_REAL Distance() { return distance; }
+  const PQP_REAL* P1() { return p1; }
+  const PQP_REAL* P2() { return p2; }
+};
+
+// Whether some triangle pair is within `tolerance` (negative = 0), depth
+// first (qsize <= 2) or best first over a queue of qsize tests, stopping at
+// the first pair found (C2A_PQP.cpp:1498-1845).  distance, p1 and p2 are set
+// when closer; as in the reference, p2 is mapped into model 2's frame either
+// way.  The reference defines C2A_Tolerance without declaring it in C2A.h,
+// so this declaration has no default qsize.  Returns PQP_OK,
+// PQP_ERR_UNPROCESSED_MODEL, or -1 when the engine reports an error.
+int C2A_Tolerance(PQP_ToleranceResult* result, PQP_REAL R1[3][3], PQP_REAL T1[3], C2A_Model* o1,
+                  PQP_REAL R2[3][3], PQP_REAL T2[3], C2A_Model* o2, PQP_REAL tolerance,
+                  int qsize);
 
 // Counts and lists (res->cont_l) the triangle pairs within `threshold` at
 // the motions' current poses (C2A.cpp:1937-1966).

@@ -34,7 +34,9 @@ typedef enum {
   C2A_ERR_HIP = 3,
   C2A_ERR_NO_DEVICE = 4,
   C2A_ERR_UNSUPPORTED = 5,
-  C2A_ERR_EMPTY_MODEL = 6   /* PQP_ERR_BUILD_EMPTY_MODEL (C2A_PQP.cpp:352-357) */
+  C2A_ERR_EMPTY_MODEL = 6,  /* PQP_ERR_BUILD_EMPTY_MODEL (C2A_PQP.cpp:352-357) */
+  C2A_ERR_CACHE = 7         /* c2a_mesh_load: not a hierarchy file, damaged, or
+                               built from other triangles */
 } c2a_status;
 
 typedef struct c2a_mesh c2a_mesh;
@@ -83,9 +85,11 @@ typedef struct {
                                than the last one (default 2; 0 = spec_depth) */
   int32_t ext_levels;       /* levels computed ahead, during the replay, below
                                replayed pairs that newly pass the descend test
-                               at the step's initial bound (default 0 = off:
-                               the extra replayed records cost more than the
-                               rounds they save on the measured workloads) */
+                               at the step's initial bound; -1 (default) = the
+                               autotuned value of each query's size bucket
+                               when spec_depth is 0 (0 under the learned
+                               order or with sample logging), else 0;
+                               0 = off */
   int32_t translation;      /* 1 (default): queries whose two rotations are
                                below 1e-8 take C2A_Solve's translation-only
                                branch (C2A.cpp:1362-1521, 2032-2049: one
@@ -159,6 +163,26 @@ int c2a_mesh_create(const double *tri_xyz9, const int32_t *tri_ids,
 int c2a_mesh_destroy(c2a_mesh *m);
 /* Number of triangles, bounding volumes and maximum depth. */
 int c2a_mesh_info(const c2a_mesh *m, int64_t *ntri, int64_t *nbv, int32_t *max_depth);
+/* c2a_mesh_create with the hierarchy built on the device (C2A_Build.cpp:
+ * 393-574 in top-down levels, one workgroup per large node): the same nodes
+ * and triangle order, bit for bit, as the host build; the device copy stays
+ * resident for the queries.  stream may be NULL. */
+int c2a_mesh_create_device(const double *tri_xyz9, const int32_t *tri_ids,
+                           const int32_t *vert_idx3, int64_t ntri, void *stream,
+                           c2a_mesh **out);
+
+/* On-disk cache of a built hierarchy (no counterpart in the reference, which
+ * rebuilds in every EndModel, C2A_PQP.cpp:331-417).  The file holds the
+ * nodes, the triangles in build order with their ids and vertex indices, a
+ * digest of the triangles, ids and vertex indices the mesh was created from,
+ * and a digest of the payload.  c2a_mesh_load with tri_xyz9 != NULL checks
+ * that those inputs (as passed to c2a_mesh_create) match the file's digest;
+ * a mismatch, a damaged or foreign file returns C2A_ERR_CACHE.  Little-endian
+ * hosts only. */
+int c2a_mesh_save(const c2a_mesh *m, const char *path);
+int c2a_mesh_load(const char *path, const double *tri_xyz9, const int32_t *tri_ids,
+                  const int32_t *vert_idx3, int64_t ntri, c2a_mesh **out);
+
 /* Caller ids (AddTri "id") of the triangles in build order. */
 int c2a_mesh_tri_ids(const c2a_mesh *m, int32_t *ids_out);
 /* Copies the built hierarchy back to the host: nodes as nbv x 32 doubles
@@ -314,6 +338,58 @@ int c2a_distance(c2a_mesh *const *meshes, int32_t nmesh, const double *poses /* 
 int c2a_collide_distance(c2a_mesh *const *meshes, int32_t nmesh, const double *poses,
                          const int32_t *mesh_pairs, const int32_t *seed_tris, double rel_err,
                          double abs_err, int64_t nq, c2a_distance_result *out, void *stream);
+
+/* C2A_Distance with qsize > 2 (C2A/src/C2A_PQP.cpp:1031-1037): the
+ * best-first search C2ADistanceQueueRecurse (:624-786) over a bounded
+ * priority queue of qsize tests; a full queue runs a nested search.  Same
+ * arguments and result as c2a_distance, which qsize <= 2 calls. */
+int c2a_distance_queue(c2a_mesh *const *meshes, int32_t nmesh, const double *poses,
+                       const int32_t *mesh_pairs, const int32_t *seed_tris, double rel_err,
+                       double abs_err, int32_t qsize, int64_t nq, c2a_distance_result *out,
+                       void *stream);
+
+/* Result of the static queries below.  Triangles are build-order indices
+ * (caller ids through c2a_mesh_tri_ids); p1 is in model 1's frame, p2 in
+ * model 2's. */
+typedef struct {
+  double distance;
+  double p1[3], p2[3];
+  int64_t bv_tests, tri_tests;
+  int32_t tri[2];       /* the pair found (contact space: the models' last_tri
+                           after the query, the seed pair when none) */
+  int32_t found;        /* contact space: a pair closer than the bound;
+                           tolerance: closer_than_tolerance; collide: the
+                           number of contacting pairs */
+  int32_t flags;        /* C2A_FLAG_STACK_OVERFLOW: the search was cut short */
+} c2a_static_result;
+
+/* C2A_Distance_At_ContactSpace (C2A/src/C2A_PQP.cpp:1418-1496 ->
+ * :1282-1415): the depth-first search from distance = upper_bounds[i] with
+ * zero error bounds and no seed pair, which stops at the first triangle pair
+ * closer than the bound.  found = 0: distance = the bound, p1/p2 unset. */
+int c2a_distance_contact_space(c2a_mesh *const *meshes, int32_t nmesh, const double *poses,
+                               const int32_t *mesh_pairs, const int32_t *seed_tris,
+                               const double *upper_bounds, int64_t nq, c2a_static_result *out,
+                               void *stream);
+
+/* C2A_Tolerance (C2A/src/C2A_PQP.cpp:1762-1845): is some triangle pair
+ * within tolerances[i] (negative = 0)?  Depth first, closer pair first
+ * (:1498-1604), or best first over a queue of qsize tests when qsize > 2
+ * (:1606-1759); stops at the first pair found.  found = closer_than_tolerance;
+ * distance, p1, p2 and tri are set only when found. */
+int c2a_tolerance(c2a_mesh *const *meshes, int32_t nmesh, const double *poses,
+                  const int32_t *mesh_pairs, const double *tolerances, int32_t qsize,
+                  int64_t nq, c2a_static_result *out, void *stream);
+
+/* C2A_Collide, PQP_CollideResult form (C2A/src/C2A_PQP.cpp:911-968 ->
+ * CollideRecurse :798-909): the intersecting triangle pairs under pairs of
+ * overlapping RSS volumes, in visiting order.  flag = 1 (C2A_ALL_CONTACTS)
+ * lists all, 2 (C2A_FIRST_CONTACT) stops at the first.  pairs_out holds nq x
+ * pair_cap x 2 build-order triangles (the first min(found, pair_cap) of
+ * query i at pairs_out + 2 * i * pair_cap); found counts them all. */
+int c2a_collide_pairs(c2a_mesh *const *meshes, int32_t nmesh, const double *poses,
+                      const int32_t *mesh_pairs, int32_t flag, int64_t nq, int32_t pair_cap,
+                      int32_t *pairs_out, c2a_static_result *out, void *stream);
 
 /* Linear motion between two poses, as the kernel integrates it
  * (CInterpMotion_Linear ctor + velocity(), InterpMotion.cpp:148-270,

@@ -2029,3 +2029,544 @@ void c2o_distance_batch(const c2o_model *m1, const c2o_model *m2, int nq, const 
   for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
   pthread_mutex_destroy(&job.mu);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Static queries beyond C2A_Distance's depth-first form                    */
+/* ------------------------------------------------------------------------ */
+
+/* Child pair transforms of an expansion (the descend rule and transforms
+ * shared by every recursion of C2A_PQP.cpp, e.g. :527-576 and :1343-1392):
+ * returns the two child pairs (a1, a2), (c1, c2) and their transforms. */
+static void expand_pair(const c2o_model *m1, const c2o_model *m2, const m3 R, const double T[3],
+                        int b1, int b2, int *a1, int *a2, int *c1, int *c2, m3 R1, double T1[3],
+                        m3 R2, double T2[3]) {
+  const c2o_bv *B1 = &m1->bvs[b1], *B2 = &m2->bvs[b2];
+  int l1 = B1->first_child < 0, l2 = B2->first_child < 0;
+  double Tt[3];
+  if (l2 || (!l1 && (B1->size > B2->size))) {
+    *a1 = B1->first_child; *c1 = *a1 + 1; *a2 = *c2 = b2;
+    const c2o_bv *A = &m1->bvs[*a1], *C = &m1->bvs[*c1];
+    mtxm(R1, A->R, R); sub3(Tt, T, A->Tr); mtxv(T1, A->R, Tt);
+    mtxm(R2, C->R, R); sub3(Tt, T, C->Tr); mtxv(T2, C->R, Tt);
+  } else {
+    *a1 = *c1 = b1; *a2 = B2->first_child; *c2 = *a2 + 1;
+    const c2o_bv *A = &m2->bvs[*a2], *C = &m2->bvs[*c2];
+    mxm(R1, R, A->R); mxvpv(T1, R, A->Tr, T);
+    mxm(R2, R, C->R); mxvpv(T2, R, C->Tr, T);
+  }
+}
+
+/* PQP's BVTQ (the bounded priority queue of BV tests behind the qsize > 2
+ * searches, C2A_PQP.cpp:624-786 and :1606-1759; PQP 1.3's BVTQ.h is not
+ * vendored, version unpinned).  Restated as a binary min-heap on d over
+ * `size` slots: AddTest sifts up past parents with a larger d, ExtractMinTest
+ * moves the last test to the root and sifts it down past the smaller child
+ * while that child's d is smaller.  Tests of equal d may leave in another
+ * order than PQP's heap gives them. */
+typedef struct {
+  m3 R;
+  double T[3];
+  double d;
+  int b1, b2;
+} c2o_bvt;
+typedef struct {
+  int size, n;
+  c2o_bvt *h;   /* h[1..n] */
+} c2o_bvtq;
+
+static void bvtq_init(c2o_bvtq *q, int size) {
+  q->size = size;
+  q->n = 0;
+  q->h = (c2o_bvt *)malloc(sizeof(c2o_bvt) * (size_t)(size + 1));
+}
+static void bvtq_free(c2o_bvtq *q) { free(q->h); }
+static void bvtq_add(c2o_bvtq *q, const c2o_bvt *t) {
+  int i = ++q->n;
+  while (i > 1 && q->h[i / 2].d > t->d) {
+    q->h[i] = q->h[i / 2];
+    i /= 2;
+  }
+  q->h[i] = *t;
+}
+static c2o_bvt bvtq_extract_min(c2o_bvtq *q) {
+  c2o_bvt top = q->h[1];
+  c2o_bvt last = q->h[q->n--];
+  int i = 1;
+  for (;;) {
+    int c = 2 * i;
+    if (c > q->n) break;
+    if (c + 1 <= q->n && q->h[c + 1].d < q->h[c].d) c++;
+    if (!(q->h[c].d < last.d)) break;
+    q->h[i] = q->h[c];
+    i = c;
+  }
+  if (q->n >= 1) q->h[i] = last;
+  return top;
+}
+
+/* The two child tests of an expansion with their RSS distances
+ * (C2A_PQP.cpp:693-771 / :1661-1746). */
+static void child_tests(const c2o_model *m1, const c2o_model *m2, const c2o_bvt *p, c2o_bvt *t1,
+                        c2o_bvt *t2) {
+  double S[3];
+  expand_pair(m1, m2, p->R, p->T, p->b1, p->b2, &t1->b1, &t1->b2, &t2->b1, &t2->b2, t1->R,
+              t1->T, t2->R, t2->T);
+  t1->d = bv_distance(t1->R, t1->T, &m1->bvs[t1->b1], &m2->bvs[t1->b2], S);
+  t2->d = bv_distance(t2->R, t2->T, &m1->bvs[t2->b1], &m2->bvs[t2->b2], S);
+}
+
+typedef struct {
+  c2o_model *m1, *m2;
+  m3 Rrel;
+  double Trel[3];
+  double rel_err, abs_err;
+  int qsize;
+  c2o_distance_result *r;
+} dqueue_ctx;
+
+/* C2ADistanceQueueRecurse (C2A_PQP.cpp:624-786). */
+static void distance_queue_recurse(dqueue_ctx *cx, const m3 R, const double T[3], int b1,
+                                   int b2) {
+  c2o_bvtq q;
+  bvtq_init(&q, cx->qsize);
+  c2o_bvt mt;
+  memcpy(mt.R, R, sizeof(m3));
+  memcpy(mt.T, T, sizeof(mt.T));
+  mt.b1 = b1;
+  mt.b2 = b2;
+  mt.d = 0.0;
+  c2o_distance_result *r = cx->r;
+  for (;;) {
+    const c2o_bv *B1 = &cx->m1->bvs[mt.b1], *B2 = &cx->m2->bvs[mt.b2];
+    int l1 = B1->first_child < 0, l2 = B2->first_child < 0;
+    if (l1 && l2) {   /* :645-669 */
+      double p[3], qq[3];
+      int t1 = -B1->first_child - 1, t2 = -B2->first_child - 1, ta, fa, tb, fb;
+      r->tri_tests++;
+      double d = tri_distance(cx->Rrel, cx->Trel, &cx->m1->tris[t1], &cx->m2->tris[t2], p, qq,
+                              &ta, &fa, &tb, &fb);
+      if (d < r->distance) {
+        r->distance = d;
+        memcpy(r->p1, p, sizeof(p));
+        memcpy(r->p2, qq, sizeof(qq));
+        cx->m1->last_tri = t1;
+        cx->m2->last_tri = t2;
+      }
+    } else if (q.n == q.size - 1) {   /* :670-676: no room for two tests */
+      distance_queue_recurse(cx, mt.R, mt.T, mt.b1, mt.b2);
+    } else {   /* :677-772 */
+      c2o_bvt t1, t2;
+      r->bv_tests += 2;
+      child_tests(cx->m1, cx->m2, &mt, &t1, &t2);
+      bvtq_add(&q, &t1);
+      bvtq_add(&q, &t2);
+    }
+    if (q.n == 0) break;   /* :774-785 */
+    mt = bvtq_extract_min(&q);
+    if ((mt.d + cx->abs_err >= r->distance) && ((mt.d * (1 + cx->rel_err)) >= r->distance))
+      break;
+  }
+  bvtq_free(&q);
+}
+
+/* C2A_Distance with qsize > 2 (C2A_PQP.cpp:970-1057, :1031-1037). */
+void c2o_distance_queue(c2o_model *m1, c2o_model *m2, const double R1[3][3], const double T1[3],
+                        const double R2[3][3], const double T2[3], double rel_err,
+                        double abs_err, int qsize, c2o_distance_result *r) {
+  if (qsize <= 2) {
+    c2o_distance(m1, m2, R1, T1, R2, T2, rel_err, abs_err, r);
+    return;
+  }
+  dqueue_ctx cx;
+  m3 R;
+  double T[3], p[3], q[3];
+  int ta, fa, tb, fb;
+  cx.m1 = m1; cx.m2 = m2; cx.rel_err = rel_err; cx.abs_err = abs_err; cx.qsize = qsize;
+  cx.r = r;
+  root_transform(m1, m2, R1, T1, R2, T2, cx.Rrel, cx.Trel, R, T);
+  r->distance = tri_distance(cx.Rrel, cx.Trel, &m1->tris[m1->last_tri], &m2->tris[m2->last_tri],
+                             p, q, &ta, &fa, &tb, &fb);
+  memcpy(r->p1, p, sizeof(p));
+  memcpy(r->p2, q, sizeof(q));
+  r->bv_tests = 0;
+  r->tri_tests = 0;
+  distance_queue_recurse(&cx, R, T, 0, 0);
+  double u[3];
+  sub3(u, r->p2, cx.Trel);
+  mtxv(r->p2, cx.Rrel, u);
+  r->last_tri[0] = m1->last_tri;
+  r->last_tri[1] = m2->last_tri;
+}
+
+/* C2A_Distance_At_ContactSpace (C2A_PQP.cpp:1282-1496): the depth-first
+ * search at zero error bounds from distance = upper_bound, no seed pair;
+ * every call returns at once after the first leaf closer than the bound
+ * (:1290).  found = 1 when that happened (the closest-pair fields are then
+ * set; otherwise distance = upper_bound and p1/p2 are not written). */
+typedef struct {
+  c2o_model *m1, *m2;
+  m3 Rrel;
+  double Trel[3];
+  double upper;
+  c2o_distance_result *r;
+} dcs_ctx;
+
+static void distance_cs_recurse(dcs_ctx *cx, const m3 R, const double T[3], int b1, int b2) {
+  c2o_distance_result *r = cx->r;
+  if (r->distance < cx->upper) return;   /* :1290 */
+  const c2o_bv *B1 = &cx->m1->bvs[b1], *B2 = &cx->m2->bvs[b2];
+  int l1 = B1->first_child < 0, l2 = B2->first_child < 0;
+  if (l1 && l2) {   /* :1299-1328 */
+    double p[3], q[3];
+    int t1 = -B1->first_child - 1, t2 = -B2->first_child - 1, ta, fa, tb, fb;
+    r->tri_tests++;
+    double d = tri_distance(cx->Rrel, cx->Trel, &cx->m1->tris[t1], &cx->m2->tris[t2], p, q,
+                            &ta, &fa, &tb, &fb);
+    if (d < r->distance) {
+      r->distance = d;
+      memcpy(r->p1, p, sizeof(p));
+      memcpy(r->p2, q, sizeof(q));
+      cx->m1->last_tri = t1;
+      cx->m2->last_tri = t2;
+    }
+    return;
+  }
+  int a1, a2, c1, c2;
+  m3 R1, R2;
+  double T1[3], T2[3], S[3];
+  expand_pair(cx->m1, cx->m2, R, T, b1, b2, &a1, &a2, &c1, &c2, R1, T1, R2, T2);
+  r->bv_tests += 2;
+  double d1 = bv_distance(R1, T1, &cx->m1->bvs[a1], &cx->m2->bvs[a2], S);
+  double d2 = bv_distance(R2, T2, &cx->m1->bvs[c1], &cx->m2->bvs[c2], S);
+  if (d2 < d1) {   /* :1391-1414 */
+    if (d2 < r->distance) distance_cs_recurse(cx, R2, T2, c1, c2);
+    if (d1 < r->distance) distance_cs_recurse(cx, R1, T1, a1, a2);
+  } else {
+    if (d1 < r->distance) distance_cs_recurse(cx, R1, T1, a1, a2);
+    if (d2 < r->distance) distance_cs_recurse(cx, R2, T2, c1, c2);
+  }
+}
+
+int c2o_distance_cs(c2o_model *m1, c2o_model *m2, const double R1[3][3], const double T1[3],
+                    const double R2[3][3], const double T2[3], double upper_bound,
+                    c2o_distance_result *r) {
+  dcs_ctx cx;
+  m3 R;
+  double T[3];
+  cx.m1 = m1; cx.m2 = m2; cx.upper = upper_bound; cx.r = r;
+  root_transform(m1, m2, R1, T1, R2, T2, cx.Rrel, cx.Trel, R, T);
+  r->distance = upper_bound;
+  r->bv_tests = 0;
+  r->tri_tests = 0;
+  r->last_tri[0] = m1->last_tri;
+  r->last_tri[1] = m2->last_tri;
+  distance_cs_recurse(&cx, R, T, 0, 0);
+  int found = r->distance < upper_bound;
+  if (found) {   /* :1484-1488 */
+    double u[3];
+    sub3(u, r->p2, cx.Trel);
+    mtxv(r->p2, cx.Rrel, u);
+  }
+  r->last_tri[0] = m1->last_tri;
+  r->last_tri[1] = m2->last_tri;
+  return found;
+}
+
+/* C2A_Tolerance (C2A_PQP.cpp:1762-1845): is any triangle pair within
+ * `tolerance`?  Depth first, closer child pair first, a pair entered while
+ * its RSS distance is <= tolerance, stopping at the first such triangle
+ * pair (:1498-1604); qsize > 2 searches best first (:1606-1759). */
+typedef struct {
+  const c2o_model *m1, *m2;
+  m3 Rrel;
+  double Trel[3];
+  double tol;
+  int qsize;
+  c2o_tolerance_result *r;
+} tol_ctx;
+
+static int tol_leaf(tol_ctx *cx, int b1, int b2) {
+  const c2o_bv *B1 = &cx->m1->bvs[b1], *B2 = &cx->m2->bvs[b2];
+  double p[3], q[3];
+  int t1 = -B1->first_child - 1, t2 = -B2->first_child - 1, ta, fa, tb, fb;
+  c2o_tolerance_result *r = cx->r;
+  r->tri_tests++;
+  double d = tri_distance(cx->Rrel, cx->Trel, &cx->m1->tris[t1], &cx->m2->tris[t2], p, q, &ta,
+                          &fa, &tb, &fb);
+  if (d <= r->tolerance) {   /* :1518-1529 */
+    r->closer = 1;
+    r->distance = d;
+    memcpy(r->p1, p, sizeof(p));
+    memcpy(r->p2, q, sizeof(q));
+    r->tri[0] = t1;
+    r->tri[1] = t2;
+    return 1;
+  }
+  return 0;
+}
+
+static void tolerance_recurse(tol_ctx *cx, const m3 R, const double T[3], int b1, int b2) {
+  const c2o_bv *B1 = &cx->m1->bvs[b1], *B2 = &cx->m2->bvs[b2];
+  c2o_tolerance_result *r = cx->r;
+  if (B1->first_child < 0 && B2->first_child < 0) {
+    tol_leaf(cx, b1, b2);
+    return;
+  }
+  int a1, a2, c1, c2;
+  m3 R1, R2;
+  double T1[3], T2[3], S[3];
+  expand_pair(cx->m1, cx->m2, R, T, b1, b2, &a1, &a2, &c1, &c2, R1, T1, R2, T2);
+  r->bv_tests += 2;
+  double d1 = bv_distance(R1, T1, &cx->m1->bvs[a1], &cx->m2->bvs[a2], S);
+  double d2 = bv_distance(R2, T2, &cx->m1->bvs[c1], &cx->m2->bvs[c2], S);
+  if (d2 < d1) {   /* :1590-1603 */
+    if (d2 <= r->tolerance) tolerance_recurse(cx, R2, T2, c1, c2);
+    if (r->closer) return;
+    if (d1 <= r->tolerance) tolerance_recurse(cx, R1, T1, a1, a2);
+  } else {
+    if (d1 <= r->tolerance) tolerance_recurse(cx, R1, T1, a1, a2);
+    if (r->closer) return;
+    if (d2 <= r->tolerance) tolerance_recurse(cx, R2, T2, c1, c2);
+  }
+}
+
+/* C2AToleranceQueueRecurse (C2A_PQP.cpp:1606-1759). */
+static void tolerance_queue_recurse(tol_ctx *cx, const m3 R, const double T[3], int b1,
+                                    int b2) {
+  c2o_bvtq q;
+  bvtq_init(&q, cx->qsize);
+  c2o_bvt mt;
+  memcpy(mt.R, R, sizeof(m3));
+  memcpy(mt.T, T, sizeof(mt.T));
+  mt.b1 = b1;
+  mt.b2 = b2;
+  mt.d = 0.0;
+  c2o_tolerance_result *r = cx->r;
+  for (;;) {
+    const c2o_bv *B1 = &cx->m1->bvs[mt.b1], *B2 = &cx->m2->bvs[mt.b2];
+    int l1 = B1->first_child < 0, l2 = B2->first_child < 0;
+    if (l1 && l2) {
+      if (tol_leaf(cx, mt.b1, mt.b2)) break;   /* :1644 */
+    } else if (q.n == q.size - 1) {   /* :1647-1654 */
+      tolerance_queue_recurse(cx, mt.R, mt.T, mt.b1, mt.b2);
+      if (r->closer == 1) break;
+    } else {
+      c2o_bvt t1, t2;
+      r->bv_tests += 2;
+      child_tests(cx->m1, cx->m2, &mt, &t1, &t2);
+      if (t1.d <= r->tolerance) bvtq_add(&q, &t1);   /* :1747-1750 */
+      if (t2.d <= r->tolerance) bvtq_add(&q, &t2);
+    }
+    if (q.n == 0 || q.h[1].d > r->tolerance) {   /* :1752-1756 */
+      r->closer = 0;
+      break;
+    }
+    mt = bvtq_extract_min(&q);
+  }
+  bvtq_free(&q);
+}
+
+void c2o_tolerance(const c2o_model *m1, const c2o_model *m2, const double R1[3][3],
+                   const double T1[3], const double R2[3][3], const double T2[3],
+                   double tolerance, int qsize, c2o_tolerance_result *r) {
+  tol_ctx cx;
+  m3 R;
+  double T[3], S[3];
+  cx.m1 = m1; cx.m2 = m2; cx.qsize = qsize; cx.r = r;
+  root_transform(m1, m2, R1, T1, R2, T2, cx.Rrel, cx.Trel, R, T);
+  if (tolerance < 0.0) tolerance = 0.0;   /* :1786 */
+  r->tolerance = tolerance;
+  r->bv_tests = 0;
+  r->tri_tests = 0;
+  r->closer = 0;
+  r->distance = 0.0;
+  r->tri[0] = r->tri[1] = -1;
+  memset(r->p1, 0, sizeof(r->p1));
+  memset(r->p2, 0, sizeof(r->p2));
+  /* the root pair's distance for a trivial reject (:1817-1819) */
+  double d = bv_distance(R, T, &m1->bvs[0], &m2->bvs[0], S);
+  if (d <= r->tolerance) {
+    if (qsize <= 2) tolerance_recurse(&cx, R, T, 0, 0);
+    else tolerance_queue_recurse(&cx, R, T, 0, 0);
+  }
+  if (r->closer) {   /* p2 into model 2's frame (:1834-1838) */
+    double u[3];
+    sub3(u, r->p2, cx.Trel);
+    mtxv(r->p2, cx.Rrel, u);
+  }
+}
+
+/* PQP's TriContact (declared at C2A_PQP.cpp:441 and C2A.cpp:37, defined in
+ * PQP 1.3's TriDist/Tri-overlap code, not vendored): the separating-axis
+ * test of two triangles over 17 candidate axes -- the two face normals,
+ * the nine edge-edge cross products and the six in-plane edge normals --
+ * with the vertices taken relative to P1.  Projections that overlap on
+ * every axis mean contact; touching counts as contact. */
+static int project6(const double ax[3], const double p1[3], const double p2[3],
+                    const double p3[3], const double q1[3], const double q2[3],
+                    const double q3[3]) {
+  double P1 = dot3(ax, p1), P2 = dot3(ax, p2), P3 = dot3(ax, p3);
+  double Q1 = dot3(ax, q1), Q2 = dot3(ax, q2), Q3 = dot3(ax, q3);
+  double mx1 = P1 > P2 ? P1 : P2, mn1 = P1 < P2 ? P1 : P2;
+  mx1 = mx1 > P3 ? mx1 : P3;
+  mn1 = mn1 < P3 ? mn1 : P3;
+  double mx2 = Q1 > Q2 ? Q1 : Q2, mn2 = Q1 < Q2 ? Q1 : Q2;
+  mx2 = mx2 > Q3 ? mx2 : Q3;
+  mn2 = mn2 < Q3 ? mn2 : Q3;
+  if (mn1 > mx2) return 0;
+  if (mn2 > mx1) return 0;
+  return 1;
+}
+
+int c2o_tri_contact(const double P1[3], const double P2[3], const double P3[3],
+                    const double Q1[3], const double Q2[3], const double Q3[3]) {
+  double p1[3] = {0, 0, 0}, p2[3], p3[3], q1[3], q2[3], q3[3];
+  double e[3][3], f[3][3], n1[3], m1[3], g[3][3], h[3][3], ef[3];
+  sub3(p2, P2, P1);
+  sub3(p3, P3, P1);
+  sub3(q1, Q1, P1);
+  sub3(q2, Q2, P1);
+  sub3(q3, Q3, P1);
+  sub3(e[0], p2, p1);
+  sub3(e[1], p3, p2);
+  sub3(e[2], p1, p3);
+  sub3(f[0], q2, q1);
+  sub3(f[1], q3, q2);
+  sub3(f[2], q1, q3);
+  cross3(n1, e[0], e[1]);
+  cross3(m1, f[0], f[1]);
+  for (int i = 0; i < 3; i++) {
+    cross3(g[i], e[i], n1);
+    cross3(h[i], f[i], m1);
+  }
+  if (!project6(n1, p1, p2, p3, q1, q2, q3)) return 0;
+  if (!project6(m1, p1, p2, p3, q1, q2, q3)) return 0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      cross3(ef, e[i], f[j]);
+      if (!project6(ef, p1, p2, p3, q1, q2, q3)) return 0;
+    }
+  for (int i = 0; i < 3; i++) {
+    if (!project6(g[i], p1, p2, p3, q1, q2, q3)) return 0;
+    if (!project6(h[i], p1, p2, p3, q1, q2, q3)) return 0;
+  }
+  return 1;
+}
+
+/* C2A_Collide, PQP_CollideResult form (C2A_PQP.cpp:911-968 ->
+ * CollideRecurse :798-909): every triangle pair in contact under pairs of
+ * overlapping RSS volumes, in visiting order (first child pair first, no
+ * distance order); flag == 2 (C2A_FIRST_CONTACT) stops at the first. */
+typedef struct {
+  const c2o_model *m1, *m2;
+  m3 Rrel;
+  double Trel[3];
+  int flag, cap;
+  int *pairs;
+  c2o_collide_result *r;
+} coll_ctx;
+
+static void collide_recurse(coll_ctx *cx, const m3 R, const double T[3], int b1, int b2) {
+  const c2o_bv *B1 = &cx->m1->bvs[b1], *B2 = &cx->m2->bvs[b2];
+  c2o_collide_result *r = cx->r;
+  r->bv_tests++;   /* :804 */
+  {   /* BV_Overlap for RSS: RectDist <= r1 + r2 (:807) */
+    double P[3], Q[3], S[3];
+    if (!(c2o_rect_dist(R, T, B1->l, B2->l, P, Q, S) <= B1->r + B2->r)) return;
+  }
+  int l1 = B1->first_child < 0, l2 = B2->first_child < 0;
+  if (l1 && l2) {   /* :813-854 */
+    int t1 = -B1->first_child - 1, t2 = -B2->first_child - 1;
+    const c2o_tri *A = &cx->m1->tris[t1], *B = &cx->m2->tris[t2];
+    double q1[3], q2[3], q3[3];
+    r->tri_tests++;
+    mxvpv(q1, cx->Rrel, B->p[0], cx->Trel);
+    mxvpv(q2, cx->Rrel, B->p[1], cx->Trel);
+    mxvpv(q3, cx->Rrel, B->p[2], cx->Trel);
+    if (c2o_tri_contact(A->p[0], A->p[1], A->p[2], q1, q2, q3)) {
+      if (r->num_pairs < cx->cap) {
+        cx->pairs[2 * r->num_pairs] = t1;
+        cx->pairs[2 * r->num_pairs + 1] = t2;
+      }
+      r->num_pairs++;
+    }
+    return;
+  }
+  int a1, a2, c1, c2;
+  m3 R1, R2;
+  double T1[3], T2[3];
+  expand_pair(cx->m1, cx->m2, R, T, b1, b2, &a1, &a2, &c1, &c2, R1, T1, R2, T2);
+  collide_recurse(cx, R1, T1, a1, a2);   /* :869-907 */
+  if (cx->flag == 2 && r->num_pairs > 0) return;
+  collide_recurse(cx, R2, T2, c1, c2);
+}
+
+void c2o_collide_pairs(const c2o_model *m1, const c2o_model *m2, const double R1[3][3],
+                       const double T1[3], const double R2[3][3], const double T2[3], int flag,
+                       int cap, int *pairs, c2o_collide_result *r) {
+  coll_ctx cx;
+  m3 R;
+  double T[3];
+  cx.m1 = m1; cx.m2 = m2; cx.flag = flag; cx.cap = cap; cx.pairs = pairs; cx.r = r;
+  root_transform(m1, m2, R1, T1, R2, T2, cx.Rrel, cx.Trel, R, T);
+  r->bv_tests = 0;
+  r->tri_tests = 0;
+  r->num_pairs = 0;
+  collide_recurse(&cx, R, T, 0, 0);
+}
+
+/* Batches of independent static queries for the tests (poses24 as in
+ * c2o_distance_batch; every query starts from the models as given). */
+void c2o_distance_queue_batch(const c2o_model *m1, const c2o_model *m2, int nq,
+                              const double *poses24, const int *seeds, double rel_err,
+                              double abs_err, int qsize, c2o_distance_result *out) {
+  c2o_model w1 = *m1, w2 = *m2;
+  for (int i = 0; i < nq; i++) {
+    m3 R1, R2;
+    double T1[3], T2[3];
+    pose_unpack(poses24 + 24 * (size_t)i, R1, T1);
+    pose_unpack(poses24 + 24 * (size_t)i + 12, R2, T2);
+    w1.last_tri = seeds ? seeds[2 * i] : 0;
+    w2.last_tri = seeds ? seeds[2 * i + 1] : 0;
+    c2o_distance_queue(&w1, &w2, R1, T1, R2, T2, rel_err, abs_err, qsize, &out[i]);
+  }
+}
+
+void c2o_distance_cs_batch(const c2o_model *m1, const c2o_model *m2, int nq,
+                           const double *poses24, const int *seeds, const double *upper,
+                           c2o_distance_result *out, int *found) {
+  c2o_model w1 = *m1, w2 = *m2;
+  for (int i = 0; i < nq; i++) {
+    m3 R1, R2;
+    double T1[3], T2[3];
+    pose_unpack(poses24 + 24 * (size_t)i, R1, T1);
+    pose_unpack(poses24 + 24 * (size_t)i + 12, R2, T2);
+    w1.last_tri = seeds ? seeds[2 * i] : 0;
+    w2.last_tri = seeds ? seeds[2 * i + 1] : 0;
+    memset(&out[i], 0, sizeof(out[i]));
+    found[i] = c2o_distance_cs(&w1, &w2, R1, T1, R2, T2, upper[i], &out[i]);
+  }
+}
+
+void c2o_tolerance_batch(const c2o_model *m1, const c2o_model *m2, int nq, const double *poses24,
+                         const double *tolerance, int qsize, c2o_tolerance_result *out) {
+  for (int i = 0; i < nq; i++) {
+    m3 R1, R2;
+    double T1[3], T2[3];
+    pose_unpack(poses24 + 24 * (size_t)i, R1, T1);
+    pose_unpack(poses24 + 24 * (size_t)i + 12, R2, T2);
+    c2o_tolerance(m1, m2, R1, T1, R2, T2, tolerance[i], qsize, &out[i]);
+  }
+}
+
+void c2o_collide_pairs_batch(const c2o_model *m1, const c2o_model *m2, int nq,
+                             const double *poses24, int flag, int cap, int *pairs,
+                             c2o_collide_result *out) {
+  for (int i = 0; i < nq; i++) {
+    m3 R1, R2;
+    double T1[3], T2[3];
+    pose_unpack(poses24 + 24 * (size_t)i, R1, T1);
+    pose_unpack(poses24 + 24 * (size_t)i + 12, R2, T2);
+    c2o_collide_pairs(m1, m2, R1, T1, R2, T2, flag, cap, pairs + 2 * (size_t)cap * i, &out[i]);
+  }
+}

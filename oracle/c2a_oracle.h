@@ -26,6 +26,19 @@
  *   static distance    pinned (brute-force minimum over all triangle pairs at
  *                      zero error bounds; error-bound envelope; closest
  *                      points at the reported distance in both frames)
+ *   distance queue     pinned (brute-force minimum at zero error bounds,
+ *                      error-bound envelope, the depth-first form's
+ *                      minimum at zero bounds); the order of equal-d
+ *                      tests in PQP's BVTQ is parity unpinned
+ *   contact-space dist pinned (brute force: found iff some pair is closer
+ *                      than the bound, the pair found is closer, the first
+ *                      improving leaf of the depth-first order)
+ *   tolerance          pinned (brute force: closer iff the minimum pair
+ *                      distance <= tolerance; reported pair within it)
+ *   TriContact         pinned (agrees with tri_dist == 0 away from touching;
+ *                      constructed piercing / coplanar-overlap / separated)
+ *   collide pairs      pinned (brute force: every contacting pair under
+ *                      overlapping volumes = every pair with tri_dist == 0)
  */
 #ifndef C2A_ORACLE_H
 #define C2A_ORACLE_H
@@ -214,6 +227,58 @@ void c2o_collide_distance(c2o_model *m1, c2o_model *m2, const double R1[3][3],
 void c2o_distance_batch(const c2o_model *m1, const c2o_model *m2, int nq, const double *poses24,
                         const int *seeds, double rel_err, double abs_err, int nthreads,
                         c2o_distance_result *out);
+
+/* C2A_Distance with qsize > 2 (C2A_PQP.cpp:1031-1037 -> the best-first
+ * search C2ADistanceQueueRecurse, :624-786) over a restatement of PQP's
+ * BVTQ heap; qsize <= 2 is c2o_distance. */
+void c2o_distance_queue(c2o_model *m1, c2o_model *m2, const double R1[3][3], const double T1[3],
+                        const double R2[3][3], const double T2[3], double rel_err,
+                        double abs_err, int qsize, c2o_distance_result *r);
+/* C2A_Distance_At_ContactSpace (C2A_PQP.cpp:1282-1496): returns 1 when a
+ * triangle pair closer than upper_bound was found (distance, p1, p2 and
+ * last_tri then describe it), else 0 with distance = upper_bound. */
+int c2o_distance_cs(c2o_model *m1, c2o_model *m2, const double R1[3][3], const double T1[3],
+                    const double R2[3][3], const double T2[3], double upper_bound,
+                    c2o_distance_result *r);
+
+/* C2A_Tolerance (C2A_PQP.cpp:1762-1845, PQP_ToleranceResult fields). */
+typedef struct {
+  int closer;           /* closer_than_tolerance */
+  int tri[2];           /* build-order triangles of the pair found, else -1 */
+  double tolerance;     /* max(tolerance, 0) */
+  double distance;      /* the pair's distance when closer, else 0 */
+  double p1[3], p2[3];  /* closest points (model 1's / model 2's frame) when closer */
+  long long bv_tests, tri_tests;
+} c2o_tolerance_result;
+void c2o_tolerance(const c2o_model *m1, const c2o_model *m2, const double R1[3][3],
+                   const double T1[3], const double R2[3][3], const double T2[3],
+                   double tolerance, int qsize, c2o_tolerance_result *r);
+
+/* C2A_Collide, PQP_CollideResult form (C2A_PQP.cpp:911-968): the first cap
+ * contacting pairs (build-order triangle indices) go to pairs[2k], pairs[2k+1]. */
+typedef struct {
+  long long bv_tests, tri_tests;
+  int num_pairs;
+  int pad;
+} c2o_collide_result;
+void c2o_collide_pairs(const c2o_model *m1, const c2o_model *m2, const double R1[3][3],
+                       const double T1[3], const double R2[3][3], const double T2[3], int flag,
+                       int cap, int *pairs, c2o_collide_result *r);
+/* PQP's TriContact: 1 when the triangles intersect or touch. */
+int c2o_tri_contact(const double P1[3], const double P2[3], const double P3[3],
+                    const double Q1[3], const double Q2[3], const double Q3[3]);
+
+void c2o_distance_queue_batch(const c2o_model *m1, const c2o_model *m2, int nq,
+                              const double *poses24, const int *seeds, double rel_err,
+                              double abs_err, int qsize, c2o_distance_result *out);
+void c2o_distance_cs_batch(const c2o_model *m1, const c2o_model *m2, int nq,
+                           const double *poses24, const int *seeds, const double *upper,
+                           c2o_distance_result *out, int *found);
+void c2o_tolerance_batch(const c2o_model *m1, const c2o_model *m2, int nq, const double *poses24,
+                         const double *tolerance, int qsize, c2o_tolerance_result *out);
+void c2o_collide_pairs_batch(const c2o_model *m1, const c2o_model *m2, int nq,
+                             const double *poses24, int flag, int cap, int *pairs,
+                             c2o_collide_result *out);
 
 /* C2A_Solve (C2A.cpp:2315-2444): tol_d = tol_t = 1e-4, contact threshold
  * 2d+0.001.  Seeds from m*->last_tri like the demo (mainTorusknot.cpp:314). */

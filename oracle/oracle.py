@@ -125,6 +125,21 @@ def lib():
         L.c2o_batch_toc.argtypes = [C.POINTER(Model), C.POINTER(Model), C.c_int, dp, dp, dp,
                                     dp, ip, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.POINTER(Result)]
+        L.c2o_distance_queue_batch.restype = None
+        L.c2o_distance_queue_batch.argtypes = [C.POINTER(Model), C.POINTER(Model), C.c_int, dp,
+                                               ip, C.c_double, C.c_double, C.c_int,
+                                               C.POINTER(DistanceResult)]
+        L.c2o_distance_cs_batch.restype = None
+        L.c2o_distance_cs_batch.argtypes = [C.POINTER(Model), C.POINTER(Model), C.c_int, dp, ip,
+                                            dp, C.POINTER(DistanceResult), ip]
+        L.c2o_tolerance_batch.restype = None
+        L.c2o_tolerance_batch.argtypes = [C.POINTER(Model), C.POINTER(Model), C.c_int, dp, dp,
+                                          C.c_int, C.POINTER(ToleranceResult)]
+        L.c2o_collide_pairs_batch.restype = None
+        L.c2o_collide_pairs_batch.argtypes = [C.POINTER(Model), C.POINTER(Model), C.c_int, dp,
+                                              C.c_int, C.c_int, ip, C.POINTER(CollideResult)]
+        L.c2o_tri_contact.restype = C.c_int
+        L.c2o_tri_contact.argtypes = [dp, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -371,6 +386,76 @@ def distance_batch(m1, m2, poses24, seeds=None, rel_err=0.0, abs_err=0.0, nthrea
                              None if sd is None else _ip(sd), float(rel_err), float(abs_err),
                              int(nthreads), out)
     return list(out)
+
+
+class ToleranceResult(C.Structure):
+    _fields_ = [("closer", C.c_int), ("tri", C.c_int * 2), ("tolerance", C.c_double),
+                ("distance", C.c_double), ("p1", C.c_double * 3), ("p2", C.c_double * 3),
+                ("bv_tests", C.c_longlong), ("tri_tests", C.c_longlong)]
+
+
+class CollideResult(C.Structure):
+    _fields_ = [("bv_tests", C.c_longlong), ("tri_tests", C.c_longlong),
+                ("num_pairs", C.c_int), ("pad", C.c_int)]
+
+
+def _poses(poses24):
+    P = f64(poses24).reshape(-1, 24)
+    return P, len(P)
+
+
+def _seeds(seeds, nq):
+    return None if seeds is None else np.ascontiguousarray(seeds, dtype=np.int32).reshape(nq, 2)
+
+
+def distance_queue_batch(m1, m2, poses24, qsize, seeds=None, rel_err=0.0, abs_err=0.0):
+    """C2A_Distance with the given qsize (> 2: the best-first queue search)."""
+    P, nq = _poses(poses24)
+    out = (DistanceResult * nq)()
+    sd = _seeds(seeds, nq)
+    lib().c2o_distance_queue_batch(C.byref(m1.m), C.byref(m2.m), nq, _dp(P),
+                                   None if sd is None else _ip(sd), float(rel_err),
+                                   float(abs_err), int(qsize), out)
+    return list(out)
+
+
+def distance_cs_batch(m1, m2, poses24, upper, seeds=None):
+    """C2A_Distance_At_ContactSpace; returns (results, found flags)."""
+    P, nq = _poses(poses24)
+    out = (DistanceResult * nq)()
+    found = np.zeros(nq, np.int32)
+    up = np.ascontiguousarray(np.broadcast_to(np.asarray(upper, np.float64), (nq,)))
+    sd = _seeds(seeds, nq)
+    lib().c2o_distance_cs_batch(C.byref(m1.m), C.byref(m2.m), nq, _dp(P),
+                                None if sd is None else _ip(sd), _dp(up), out, _ip(found))
+    return list(out), found
+
+
+def tolerance_batch(m1, m2, poses24, tolerance, qsize=2):
+    """C2A_Tolerance per query (tolerance: scalar or one per query)."""
+    P, nq = _poses(poses24)
+    out = (ToleranceResult * nq)()
+    tol = np.ascontiguousarray(np.broadcast_to(np.asarray(tolerance, np.float64), (nq,)))
+    lib().c2o_tolerance_batch(C.byref(m1.m), C.byref(m2.m), nq, _dp(P), _dp(tol), int(qsize),
+                              out)
+    return list(out)
+
+
+def collide_pairs_batch(m1, m2, poses24, flag=1, cap=1024):
+    """C2A_Collide's PQP_CollideResult form; returns (results, pairs (nq, cap, 2))."""
+    P, nq = _poses(poses24)
+    out = (CollideResult * nq)()
+    pairs = np.full((nq, cap, 2), -1, np.int32)
+    lib().c2o_collide_pairs_batch(C.byref(m1.m), C.byref(m2.m), nq, _dp(P), int(flag), int(cap),
+                                  _ip(pairs), out)
+    return list(out), pairs
+
+
+def tri_contact(A, B):
+    """PQP's TriContact on two (3, 3) vertex arrays in one frame."""
+    A, B = f64(A).reshape(3, 3), f64(B).reshape(3, 3)
+    return lib().c2o_tri_contact(_dp(A[0]), _dp(A[1]), _dp(A[2]), _dp(B[0]), _dp(B[1]),
+                                 _dp(B[2]))
 
 
 def toc_step(m1, m2, mo1, mo2, R1, T1, R2, T2, seeds, numCA, mint_prev, upbound):

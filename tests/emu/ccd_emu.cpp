@@ -19,7 +19,14 @@ static int64_t g_kind[2][4];
       abort();                                                               \
     }                                                                        \
   } while (0)   // rounds by reason, first step / later steps
+#ifdef C2A_EMU_SIMT
+// 64 lanes on fibers (simt.cpp); rounds are counted once, by lane 0
+namespace c2a { namespace simt { int lane_id(); int launch(void (*)(int, void*), void*); } }
+#define C2A_EMU_COUNT(kind, L) \
+  (c2a::simt::lane_id() == 0 ? (void)g_kind[(L).numCA > 1 ? 1 : 0][kind]++ : (void)0)
+#else
 #define C2A_EMU_COUNT(kind, L) (g_kind[(L).numCA > 1 ? 1 : 0][kind]++)
+#endif
 #include "../../c2a_amd/csrc/bvh.h"
 #include "../../c2a_amd/csrc/ccd_core.h"
 
@@ -76,6 +83,54 @@ extern "C" int64_t emu_samples(float* rows, int64_t* keys, int64_t cap) {
   return (int64_t)g_scount;
 }
 
+#ifdef C2A_EMU_SIMT
+// One wavefront of 64 lanes running the kernel's loop (ccd_kernel.hip), the
+// learned order's scores computed per lane by the scalar forward pass.
+template <int K, int CAP>
+struct SimtCtx {
+  const KArgs* a;
+  Wave<K, CAP>* g;
+  unsigned char* ab;
+};
+
+template <int K, int CAP>
+static void simt_body(int lane, void* p) {
+  SimtCtx<K, CAP>& c = *(SimtCtx<K, CAP>*)p;
+  const KArgs& a = *c.a;
+  Wave<K, CAP>& g = *c.g;
+  Leader L;
+  memset(&L, 0, sizeof(L));
+  L.state = ST_FETCH;
+  L.ab = c.ab;
+  for (;;) {
+    leader_run(g, L, a, lane, 64);
+    if (L.state == ST_DONE) break;
+    if (L.need_score) {
+      const ArenaView av = arena_view(L.ab, a.logcap, true);
+      for (int r = 1 + lane; r < 1 + 2 * L.nlog_prev; r += 64)
+        av.hot[r].score = score_dev_scalar(a.scorer, av.feat + (int64_t)r * NF);
+      mem_sync();
+      continue;
+    }
+    if (lane == 0) g_rounds++;
+    round_lane(g, L, lane, 64, a);
+    if (g.rarena) mem_sync();
+    else lane_sync();
+  }
+}
+
+template <int K, int CAP>
+static int run(const KArgs& a) {
+  std::vector<unsigned char> arena(a.logcap > 0 ? arena_bytes(a.logcap, a.feats != 0) : 1);
+  std::unique_ptr<Wave<K, CAP>> gp(new Wave<K, CAP>());
+  memset(gp.get(), 0, sizeof(Wave<K, CAP>));
+  g_rounds = 0;
+  memset(g_kind, 0, sizeof(g_kind));
+  g_scount = 0;
+  SimtCtx<K, CAP> ctx{&a, gp.get(), a.logcap > 0 ? arena.data() : nullptr};
+  return simt::launch(simt_body<K, CAP>, &ctx) ? 10 : 0;
+}
+#else
 template <int K, int CAP>
 static int run(const KArgs& a) {
   std::vector<unsigned char> arena(a.logcap > 0 ? arena_bytes(a.logcap, a.feats != 0) : 1);
@@ -104,6 +159,7 @@ static int run(const KArgs& a) {
   }
   return 0;
 }
+#endif
 
 extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris2, int64_t n2,
                              const c2a_query* q, int64_t nq, const c2a_params* p, c2a_result* out,
@@ -149,7 +205,7 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
   }
   a.arena_rounds = emu_arena_rounds;
   a.miss_depth = p->miss_depth;
-  a.ext_levels = p->ext_levels;
+  a.ext_levels = p->ext_levels < 0 ? 0 : p->ext_levels;
   a.strict = p->strict != 0;
   a.translation = p->translation && !a.strict;
   a.scorer = g_scorer;
@@ -214,4 +270,53 @@ extern "C" int emu_distance(const double* tris1, int64_t n1, const double* tris2
   a.mode = MODE_DISTANCE;
   const bool small = b1.max_depth + b2.max_depth + 2 <= 48;
   return small ? run<5, 48>(a) : run<5, 96>(a);
+}
+
+// Static queries of static_query.h (one thread's code, run query by query)
+// between two meshes; pairs_out as c2a_collide_pairs.
+#include "../../c2a_amd/csrc/static_query.h"
+extern "C" int emu_static(const double* tris1, int64_t n1, const double* tris2, int64_t n2,
+                          int kind, const double* poses, const int32_t* seeds,
+                          const double* params, double rel_err, double abs_err, int qsize,
+                          int flag, int64_t nq, int pair_cap, int32_t* pairs_out,
+                          c2a::StaticOut* out) {
+  Bvh b1, b2;
+  if (build_bvh(tris1, nullptr, nullptr, n1, &b1) || build_bvh(tris2, nullptr, nullptr, n2, &b2))
+    return 1;
+  MeshDev tab[2];
+  tab[0].nodes = b1.nodes.data(); tab[0].tris = b1.tris.data();
+  tab[0].ntri = (int32_t)n1; tab[0].depth = b1.max_depth;
+  tab[1].nodes = b2.nodes.data(); tab[1].tris = b2.tris.data();
+  tab[1].ntri = (int32_t)n2; tab[1].depth = b2.max_depth;
+  std::vector<StaticQueryDev> hq((size_t)nq);
+  for (int64_t i = 0; i < nq; i++) {
+    StaticQueryDev& q = hq[(size_t)i];
+    memset(&q, 0, sizeof(q));
+    q.mesh[0] = 0;
+    q.mesh[1] = 1;
+    q.seed[0] = seeds ? seeds[2 * i] : 0;
+    q.seed[1] = seeds ? seeds[2 * i + 1] : 0;
+    memcpy(q.R1, poses + 24 * i, 72); memcpy(q.T1, poses + 24 * i + 9, 24);
+    memcpy(q.R2, poses + 24 * i + 12, 72); memcpy(q.T2, poses + 24 * i + 21, 24);
+    q.param = params ? params[i] : 0.0;
+  }
+  const int64_t per = sq_ws_entries(kind, qsize, b1.max_depth + b2.max_depth);
+  std::vector<SqEntry> ws((size_t)per);
+  StaticArgs a;
+  memset(&a, 0, sizeof(a));
+  a.meshes = tab;
+  a.q = hq.data();
+  a.out = out;
+  a.pairs = pairs_out;
+  a.ws = ws.data();
+  a.nq = nq;
+  a.ws_entries = per;
+  a.kind = kind;
+  a.qsize = qsize;
+  a.flag = flag;
+  a.pair_cap = pair_cap;
+  a.rel_err = rel_err;
+  a.abs_err = abs_err;
+  for (int64_t i = 0; i < nq; i++) static_query(a, i, ws.data());
+  return 0;
 }

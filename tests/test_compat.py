@@ -213,3 +213,82 @@ def test_solve_reused_result_translation_only(tmp_path):
         assert (g["i5"][3], g["i5"][4]) == (0, 0), i          # models untouched
         assert (g["i5"][5], g["i5"][6]) == (res.out_last_triA, res.out_last_triB), i
         seed = (res.out_last_triA, res.out_last_triB)
+
+
+def _static_payload(t1, t2, kind, qf, param, poses):
+    return (_i32(len(t1)) + t1.tobytes() + _i32(len(t2)) + t2.tobytes() + _i32(kind) + _i32(qf) +
+            np.array([param], np.float64).tobytes() + _i32(len(poses)) +
+            np.ascontiguousarray(poses).tobytes())
+
+
+@pytest.mark.gpu
+def test_static_queries_through_compat(tmp_path):
+    """C2A_Distance with qsize 6, C2A_Distance_At_ContactSpace, C2A_Tolerance
+    (depth and best first) and C2A_Collide's PQP_CollideResult form through
+    the C++ layer, the models' last_tri carried between calls: every output
+    equal to the oracle's run of the same sequence (caller ids are
+    1000 + the AddTri order)."""
+    tris, _ = W.torus_knot(n_seg=96, n_ring=10)
+    t9 = np.ascontiguousarray(tris.reshape(-1, 9))
+    nq = 24
+    P = W.knot_queries(nq, 717, collide_frac=0.5)
+    poses = np.ascontiguousarray(np.concatenate([P[0], P[2]], axis=1))
+    m1, m2 = O.OModel(t9), O.OModel(t9)
+    ids = np.array(m1.tris()["id"]) + 1000
+
+    # C2A_Distance, qsize 6 (best first), then the contact-space distance
+    for kind, qf, param in [(0, 6, 0.0), (1, 0, 0.05)]:
+        print("static_demo kind %d: %d queries" % (kind, nq), flush=True)
+        b = _run("static_demo", _static_payload(t9, t9, kind, qf, param, poses), tmp_path)
+        got = np.frombuffer(b, np.dtype([("d7", "<f8", 7), ("i6", "<i4", 6)]))
+        assert len(got) == nq
+        m1.last_tri, m2.last_tri = 0, 0
+        for i in range(nq):
+            seeds = np.array([[m1.last_tri, m2.last_tri]], np.int32)
+            if kind == 0:
+                r = O.distance_queue_batch(m1, m2, poses[i:i + 1], qf, seeds=seeds)[0]
+                found = True
+            else:
+                rr, fd = O.distance_cs_batch(m1, m2, poses[i:i + 1], param, seeds=seeds)
+                r, found = rr[0], bool(fd[0])
+            m1.last_tri, m2.last_tri = r.last_tri[0], r.last_tri[1]
+            g = got[i]
+            assert g["d7"][0] == r.distance, (kind, i)
+            if found:
+                assert tuple(g["d7"][1:4]) == tuple(r.p1), (kind, i)
+                assert tuple(g["d7"][4:]) == tuple(r.p2), (kind, i)
+                assert (g["i6"][2], g["i6"][3]) == (ids[r.last_tri[0]], ids[r.last_tri[1]])
+            assert (g["i6"][0], g["i6"][1]) == (r.bv_tests, r.tri_tests), (kind, i)
+            assert (g["i6"][4], g["i6"][5]) == (r.last_tri[0], r.last_tri[1]), (kind, i)
+
+    # C2A_Tolerance, depth first and best first
+    for qs in (2, 5):
+        print("static_demo tolerance qsize %d" % qs, flush=True)
+        b = _run("static_demo", _static_payload(t9, t9, 2, qs, 0.02, poses), tmp_path)
+        got = np.frombuffer(b, np.dtype([("i3", "<i4", 3), ("d7", "<f8", 7)]))
+        ref = O.tolerance_batch(m1, m2, poses, 0.02, qs)
+        for i, r in enumerate(ref):
+            g = got[i]
+            assert tuple(g["i3"]) == (r.closer, r.bv_tests, r.tri_tests), (qs, i)
+            if r.closer:
+                assert g["d7"][0] == r.distance
+                assert tuple(g["d7"][1:4]) == tuple(r.p1) and tuple(g["d7"][4:]) == tuple(r.p2)
+
+    # C2A_Collide, all contacts then the first contact
+    for flag in (1, 2):
+        print("static_demo collide flag %d" % flag, flush=True)
+        b = _run("static_demo", _static_payload(t9, t9, 3, flag, 0.0, poses), tmp_path)
+        ref, pairs = O.collide_pairs_batch(m1, m2, poses, flag=flag, cap=1 << 14)
+        words = np.frombuffer(b, "<i4")
+        k = 0
+        hits = 0
+        for i, r in enumerate(ref):
+            n, bv, tri = words[k:k + 3]
+            k += 3
+            assert (n, bv, tri) == (r.num_pairs, r.bv_tests, r.tri_tests), (flag, i)
+            got_pairs = words[k:k + 2 * n].reshape(-1, 2)
+            k += 2 * n
+            want = ids[pairs[i, :n]]
+            assert np.array_equal(got_pairs, want), (flag, i)
+            hits += n > 0
+        assert k == len(words) and hits > 0

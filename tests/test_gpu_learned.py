@@ -28,7 +28,11 @@ def test_gpu_samples_match_cpu_harness():
     """The GPU logs the same samples (keys) as the kernel logic run on the
     CPU; features agree to BF16 rounding, labels to FP32 rounding."""
     t9, meshes, P = small_knot(12, 31)
-    params = G.default_params(tol_d=1e-3, tol_t=1e-4, sample_rate=1.0, sample_cap=300000)
+    # the same traversal knobs on both sides: samples come from the replayed
+    # records, whose set depends on the replay log and its extension
+    knobs = dict(tol_d=1e-3, tol_t=1e-4, spec_depth=5, miss_depth=3, replay_log=8192,
+                 ext_levels=0)
+    params = G.default_params(sample_rate=1.0, sample_cap=300000, **knobs)
     b = G.Batch(meshes, G.make_queries(*P), params)
     b.run()
     X, y, k, n = b.samples(300000)
@@ -40,7 +44,7 @@ def test_gpu_samples_match_cpu_harness():
     emu.emu_set_sampling(1.0, 300000)
     try:
         from tests.test_kernel_logic_cpu import run_emu
-        run_emu(emu, t9, t9, P, G.default_params(tol_d=1e-3, tol_t=1e-4), 5)
+        run_emu(emu, t9, t9, P, G.default_params(**knobs), 5)
         rows = np.zeros((300000, G.NF + 1), np.float32)
         keys = np.zeros(300000, np.int64)
         m = emu.emu_samples(rows.ctypes.data, keys.ctypes.data, 300000)
