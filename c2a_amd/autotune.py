@@ -15,7 +15,9 @@ and, for the bucket above 131,072 triangles, the grazing batch between two
 
 (depth and miss depth first at the bucket's default log without extension,
 then the log at the best pair, then the extension levels at the best
-triple), median of three kernel timings after a warm-up.  Every
+triple), median of three kernel timings after a warm-up.  A larger log or
+more extension levels cost arena memory and traffic, so they are taken only
+when at least MARGIN faster than the best smaller setting.  Every
 setting gives the same results bit for bit; only the time differs.  The
 chosen knobs go to tuning.json, which c2a_amd/build.py compiles into the
 library: c2a_batch_create applies them per query when spec_depth is 0.
@@ -35,6 +37,7 @@ from c2a_amd import workloads as W
 _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_TABLE = os.path.join(_HERE, "tuning.json")
 BUCKETS = [128, 512, 2048, 8192, 32768, 131072, 1 << 30]
+MARGIN = 0.02   # relative gain a larger replay log or extension must bring
 
 
 def bucket_of(ntri):
@@ -82,6 +85,8 @@ def tune_bucket(meshes, queries, tol_d, tol_t, max_ntri, depths, misses, logs, e
             log("  depth %d miss %d log %d: %.2f ms" % (d, m, base_log, ms))
             if best is None or ms < best["ms"]:
                 best = trials[-1]
+    # logs in increasing size: a larger one must beat the best so far by MARGIN
+    by_log = [best]
     for lg in logs:
         if lg == base_log:
             continue
@@ -92,9 +97,13 @@ def tune_bucket(meshes, queries, tol_d, tol_t, max_ntri, depths, misses, logs, e
         trials.append({"spec_depth": best["spec_depth"], "miss_depth": best["miss_depth"],
                        "replay_log": lg, "ext_levels": 0, "ms": ms})
         log("  depth %d miss %d log %d: %.2f ms" % (best["spec_depth"], best["miss_depth"], lg, ms))
-        if ms < best["ms"]:
-            best = trials[-1]
-    for ext in exts:
+        by_log.append(trials[-1])
+    pick = None
+    for t in sorted(by_log, key=lambda t: t["replay_log"]):
+        if pick is None or t["ms"] < pick["ms"] * (1 - MARGIN):
+            pick = t
+    best = pick
+    for ext in sorted(exts):
         if ext == 0:
             continue
         bd, bm, bl = best["spec_depth"], best["miss_depth"], best["replay_log"]
@@ -104,7 +113,7 @@ def tune_bucket(meshes, queries, tol_d, tol_t, max_ntri, depths, misses, logs, e
         trials.append({"spec_depth": bd, "miss_depth": bm, "replay_log": bl, "ext_levels": ext,
                        "ms": ms})
         log("  depth %d miss %d log %d ext %d: %.2f ms" % (bd, bm, bl, ext, ms))
-        if ms < best["ms"]:
+        if ms < best["ms"] * (1 - MARGIN):
             best = trials[-1]
     return best, trials
 

@@ -2,16 +2,17 @@
 // builder's (bvh.cpp), which follows the reference's top-down build
 // (C2A_Build.cpp:393-574, C2A_BV.cpp:71-111 and 354-635).
 //
-// Top levels: one workgroup per node, level by level.  The workgroup stages
-// the node's triangles through LDS and one lane runs the covariance sums in
-// triangle order (sums are order-dependent; they are the only sequential
-// part); every minimum, maximum and arg-extreme of the fit is reduced across
-// the workgroup (exact in any order); the corner pass runs sequentially over
-// the few points that can grow a corner; and the in-place partition of the
-// split is reproduced in parallel from prefix counts (see the split below).
-// Nodes of at most kSmallNode triangles are finished, subtree and all, by
-// one thread each with the host's sequential code.  Node numbers follow the
-// host's depth-first order through dfs_first_child.
+// Level by level, every node of a level at once, in three sizes: a
+// 256-thread workgroup per node above kWaveMax triangles, one wavefront per
+// node down to kThreadMax, one thread per node below.  A workgroup (or
+// wavefront) stages the node's triangles through LDS and one lane runs the
+// covariance sums in triangle order (sums are order-dependent; they are the
+// only sequential part); every minimum, maximum and arg-extreme of the fit is
+// reduced across the lanes (exact in any order); the corner pass runs
+// sequentially over the few points that can grow a corner; and the in-place
+// partition of the split is reproduced in parallel from prefix counts (see
+// the split below).  A one-thread node runs the host's sequential code.
+// Node numbers follow the host's depth-first order through dfs_first_child.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -23,8 +24,13 @@ namespace c2a {
 
 namespace {
 
-constexpr int kBlock = 256;      // threads per workgroup of the top levels
-constexpr int kSmallNode = 64;   // nodes at most this large finish on one thread
+constexpr int kWaveMax = 2048;   // nodes above this many triangles: 256 threads
+constexpr int kThreadMax = 24;   // nodes at most this large: one thread
+
+// node classes: 0 workgroup, 1 wavefront, 2 thread
+__host__ __device__ inline int node_class(int32_t num) {
+  return num > kWaveMax ? 0 : (num > kThreadMax ? 1 : 2);
+}
 
 struct BuildJob {
   int32_t bn, first, num, depth, rights, pad[3];
@@ -37,12 +43,15 @@ struct BuildArgs {
   int32_t* tape;           // 2n: the partition's queue tape
   int32_t* cand;           // 3n: corner candidates
   GpuNode* nodes;          // 2n - 1, zeroed
-  const BuildJob* jobs;    // this level's large jobs
-  BuildJob* next_large;    // the next level's large jobs
-  BuildJob* small;         // jobs finished by build_small
-  int32_t* counts;         // [0] next large, [1] small, [2] max depth
-  int32_t njobs;
+  const BuildJob* jobs[3];  // this level's nodes of each class
+  BuildJob* next[3];        // the next level's nodes of each class
+  int32_t* counts;          // [0..2] next level's node counts, [3] max depth
 };
+
+__device__ inline void push_child(const BuildArgs& a, const BuildJob& j) {
+  const int c = node_class(j.num);
+  a.next[c][atomicAdd(&a.counts[c], 1)] = j;
+}
 
 __device__ inline const double* tri_at(const BuildArgs& a, int32_t pos) {
   return a.src + 9 * (int64_t)a.perm[pos];
@@ -54,8 +63,9 @@ __device__ inline void proj(const BuildArgs& a, const double* R, int32_t first, 
   mtxv(P, R, v);
 }
 
-// ---- workgroup reductions (kBlock = 4 wavefronts) --------------------------
+// ---- workgroup reductions over B threads (B / 64 wavefronts) --------------
 
+template <int B>
 __device__ double block_max(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) {
     const double w = __shfl_xor(v, o);
@@ -64,11 +74,12 @@ __device__ double block_max(double v, double* sh) {
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
   double r = sh[0];
-  for (int k = 1; k < kBlock / 64; k++) r = sh[k] > r ? sh[k] : r;
+  for (int k = 1; k < B / 64; k++) r = sh[k] > r ? sh[k] : r;
   __syncthreads();
   return r;
 }
 
+template <int B>
 __device__ double block_min(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) {
     const double w = __shfl_xor(v, o);
@@ -77,13 +88,14 @@ __device__ double block_min(double v, double* sh) {
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
   double r = sh[0];
-  for (int k = 1; k < kBlock / 64; k++) r = sh[k] < r ? sh[k] : r;
+  for (int k = 1; k < B / 64; k++) r = sh[k] < r ? sh[k] : r;
   __syncthreads();
   return r;
 }
 
 // (value, index) with the smallest value, first index among ties; sign = -1
 // turns it into the largest value
+template <int B>
 __device__ int block_argext(double v, int idx, double sign, double* shv, int* shi) {
   double key = sign * v;
   for (int o = 32; o > 0; o >>= 1) {
@@ -101,7 +113,7 @@ __device__ int block_argext(double v, int idx, double sign, double* shv, int* sh
   __syncthreads();
   double bk = shv[0];
   int bi = shi[0];
-  for (int k = 1; k < kBlock / 64; k++)
+  for (int k = 1; k < B / 64; k++)
     if (shv[k] < bk || (shv[k] == bk && shi[k] < bi)) {
       bk = shv[k];
       bi = shi[k];
@@ -111,6 +123,7 @@ __device__ int block_argext(double v, int idx, double sign, double* shv, int* sh
 }
 
 // exclusive prefix sum of v over the workgroup; total gets the sum
+template <int B>
 __device__ int block_scan(int v, int* sh, int& total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int x = v;
@@ -122,7 +135,7 @@ __device__ int block_scan(int v, int* sh, int& total) {
   __syncthreads();
   int before = 0;
   total = 0;
-  for (int k = 0; k < kBlock / 64; k++) {
+  for (int k = 0; k < B / 64; k++) {
     if (k < w) before += sh[k];
     total += sh[k];
   }
@@ -132,14 +145,15 @@ __device__ int block_scan(int v, int* sh, int& total) {
 
 // ---- one large node per workgroup -----------------------------------------
 
-__global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
-  __shared__ double chunk[kBlock * 9];
-  __shared__ double shv[kBlock / 64];
-  __shared__ int shi[kBlock / 64];
-  __shared__ int shs[kBlock / 64];
+template <int B>
+__global__ void __launch_bounds__(B) build_node(BuildArgs a, int cls) {
+  __shared__ double chunk[B * 9];
+  __shared__ double shv[B / 64];
+  __shared__ int shi[B / 64];
+  __shared__ int shs[B / 64];
   __shared__ double acc[9], R[9], box[4];
   __shared__ int flag;
-  const BuildJob job = a.jobs[blockIdx.x];
+  const BuildJob job = a.jobs[cls][blockIdx.x];
   const int tid = threadIdx.x;
   const int32_t first = job.first, num = job.num;
   const int np = 3 * num;
@@ -147,7 +161,7 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
   // covariance sums, in triangle order (C2A_Build.cpp:279-335)
   if (tid == 0)
     for (int k = 0; k < 9; k++) acc[k] = 0.0;
-  for (int base = 0; base < num; base += kBlock) {
+  for (int base = 0; base < num; base += B) {
     const int i = base + tid;
     if (i < num) {
       const double* t = tri_at(a, first + i);
@@ -155,7 +169,7 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
     }
     __syncthreads();
     if (tid == 0) {
-      const int m = num - base < kBlock ? num - base : kBlock;
+      const int m = num - base < B ? num - base : B;
       double s[9];
       for (int k = 0; k < 9; k++) s[k] = acc[k];
       for (int j = 0; j < m; j++) cov_add(s, chunk + 9 * j);
@@ -179,7 +193,7 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
   double amax = 0.0, zmin = INFINITY, zmax = -INFINITY;
   double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY;
   int ixmn = 0x7fffffff, ixmx = 0x7fffffff, iymn = 0x7fffffff, iymx = 0x7fffffff;
-  for (int i = tid; i < np; i += kBlock) {
+  for (int i = tid; i < np; i += B) {
     const double* v = tri_at(a, first + i / 3) + 3 * (i % 3);
     const double d = vertex_radius(v);
     amax = d > amax ? d : amax;
@@ -192,13 +206,13 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
     if (P[1] < ymn) { ymn = P[1]; iymn = i; }
     if (P[1] > ymx) { ymx = P[1]; iymx = i; }
   }
-  const double angR = block_max(amax, shv);
-  const double minz = block_min(zmin, shv);
-  const double maxz = block_max(zmax, shv);
-  ixmn = block_argext(xmn, ixmn, 1.0, shv, shi);
-  ixmx = block_argext(xmx, ixmx, -1.0, shv, shi);
-  iymn = block_argext(ymn, iymn, 1.0, shv, shi);
-  iymx = block_argext(ymx, iymx, -1.0, shv, shi);
+  const double angR = block_max<B>(amax, shv);
+  const double minz = block_min<B>(zmin, shv);
+  const double maxz = block_max<B>(zmax, shv);
+  ixmn = block_argext<B>(xmn, ixmn, 1.0, shv, shi);
+  ixmx = block_argext<B>(xmx, ixmx, -1.0, shv, shi);
+  iymn = block_argext<B>(ymn, iymn, 1.0, shv, shi);
+  iymx = block_argext<B>(ymx, iymx, -1.0, shv, shi);
   const double r = 0.5 * (maxz - minz);
   const double radsqr = r * r;
   const double cz = 0.5 * (maxz + minz);
@@ -212,20 +226,20 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
   proj(a, Rr, first, iymx, P);
   const double hiy0 = extent_hi(P[1], P[2], cz, radsqr);
   double lox = lox0, hix = hix0, loy = loy0, hiy = hiy0;
-  for (int i = tid; i < np; i += kBlock) {
+  for (int i = tid; i < np; i += B) {
     proj(a, Rr, first, i, P);
     if (P[0] < lox0) { const double x = extent_lo(P[0], P[2], cz, radsqr); lox = x < lox ? x : lox; }
     if (P[0] > hix0) { const double x = extent_hi(P[0], P[2], cz, radsqr); hix = x > hix ? x : hix; }
     if (P[1] < loy0) { const double x = extent_lo(P[1], P[2], cz, radsqr); loy = x < loy ? x : loy; }
     if (P[1] > hiy0) { const double x = extent_hi(P[1], P[2], cz, radsqr); hiy = x > hiy ? x : hiy; }
   }
-  double minx = block_min(lox, shv), maxx = block_max(hix, shv);
-  double miny = block_min(loy, shv), maxy = block_max(hiy, shv);
+  double minx = block_min<B>(lox, shv), maxx = block_max<B>(hix, shv);
+  double miny = block_min<B>(loy, shv), maxy = block_max<B>(hiy, shv);
 
   // corner pass: the candidates in point order, then one lane in sequence
   int ncand = 0;
   int32_t* cand = a.cand + 3 * (int64_t)first;
-  for (int base = 0; base < np; base += kBlock) {
+  for (int base = 0; base < np; base += B) {
     const int i = base + tid;
     int f = 0;
     if (i < np) {
@@ -233,7 +247,7 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
       f = corner_candidate(P[0], P[1], minx, maxx, miny, maxy) ? 1 : 0;
     }
     int tot;
-    const int at = block_scan(f, shs, tot);
+    const int at = block_scan<B>(f, shs, tot);
     if (f) cand[ncand + at] = i;
     ncand += tot;
   }
@@ -250,7 +264,7 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
     for (int k = 0; k < 9; k++) nd.Rloc[k] = Rr[k];
     nd.depth = job.depth;
     nd.first_child = dfs_first_child(job.depth, first, job.rights);
-    atomicMax(&a.counts[2], job.depth);
+    atomicMax(&a.counts[3], job.depth);
     const double ax[3] = {Rr[0], Rr[3], Rr[6]};
     double s9[9];
     for (int k = 0; k < 9; k++) s9[k] = acc[k];
@@ -269,7 +283,7 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
   const double coord = box[0];
   int32_t* tape = a.tape + 2 * (int64_t)first;
   int nsmall = 0, nbig = 0, ncyc = 0;
-  for (int base = 0; base < num; base += kBlock) {
+  for (int base = 0; base < num; base += B) {
     const int i = base + tid;
     int s = 0, b = 0;
     if (i < num) {
@@ -277,11 +291,11 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
       b = 1 - s;
     }
     int tot;
-    const int ex = block_scan(s | (b << 16), shs, tot);
+    const int ex = block_scan<B>(s | (b << 16), shs, tot);
     const int sb = nsmall + (ex & 0xffff), bb = nbig + (ex >> 16);
     const int cyc = (s && bb > 0) ? 1 : 0;
     int ctot;
-    const int cb = ncyc + block_scan(cyc, shs, ctot);
+    const int cb = ncyc + block_scan<B>(cyc, shs, ctot);
     if (i < num) {
       const int32_t elem = a.perm[first + i];
       if (s) a.perm_tmp[first + sb] = elem;
@@ -297,7 +311,7 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
   for (;;) {
     if (tid == 0) flag = 0;
     __syncthreads();
-    for (int p = tid; p < T; p += kBlock) {
+    for (int p = tid; p < T; p += B) {
       const int32_t v = tape[p];
       if (v < 0) {
         tape[p] = tape[-v - 1];
@@ -308,9 +322,9 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
     if (!flag) break;
     __syncthreads();
   }
-  for (int j = tid; j < nbig; j += kBlock) a.perm_tmp[first + nsmall + j] = tape[ncyc + j];
+  for (int j = tid; j < nbig; j += B) a.perm_tmp[first + nsmall + j] = tape[ncyc + j];
   __syncthreads();
-  for (int i = tid; i < num; i += kBlock) a.perm[first + i] = a.perm_tmp[first + i];
+  for (int i = tid; i < num; i += B) a.perm[first + i] = a.perm_tmp[first + i];
   if (tid == 0) {
     int c1 = nsmall;
     if (c1 == 0 || c1 == num) c1 = num / 2;
@@ -318,10 +332,8 @@ __global__ void __launch_bounds__(kBlock) build_large(BuildArgs a) {
     BuildJob kids[2];
     kids[0] = BuildJob{fc, first, c1, job.depth + 1, job.rights, {0, 0, 0}};
     kids[1] = BuildJob{fc + 1, first + c1, num - c1, job.depth + 1, job.rights + 1, {0, 0, 0}};
-    for (int k = 0; k < 2; k++) {
-      if (kids[k].num > kSmallNode) a.next_large[atomicAdd(&a.counts[0], 1)] = kids[k];
-      else a.small[atomicAdd(&a.counts[1], 1)] = kids[k];
-    }
+    push_child(a, kids[0]);
+    push_child(a, kids[1]);
   }
 }
 
@@ -395,41 +407,34 @@ __device__ void fit_small(const BuildArgs& a, const BuildJob& j, double* S1) {
   nd.depth = j.depth;
 }
 
-__global__ void __launch_bounds__(64) build_small(BuildArgs a, int32_t nsmall) {
+__global__ void __launch_bounds__(64) build_thread(BuildArgs a, int32_t njobs) {
   const int t = blockIdx.x * 64 + threadIdx.x;
-  if (t >= nsmall) return;
-  BuildJob st[kSmallNode + 2];   // pending siblings of the path, plus two
-  int sp = 0;
-  st[sp++] = a.small[t];
-  int maxd = 0;
-  while (sp > 0) {
-    const BuildJob j = st[--sp];
-    double S1[3];
-    fit_small(a, j, S1);
-    GpuNode& nd = a.nodes[j.bn];
-    if (j.depth > maxd) maxd = j.depth;
-    if (j.num == 1) {
-      nd.first_child = -(j.first + 1);
-      continue;
-    }
-    const int fc = dfs_first_child(j.depth, j.first, j.rights);
-    nd.first_child = fc;
-    const double axis[3] = {nd.R[0], nd.R[3], nd.R[6]};
-    const double coord = split_plane(S1, j.num, axis);
-    int c1 = 0;
-    for (int i = 0; i < j.num; i++) {
-      if (split_coord(tri_at(a, j.first + i), axis) <= coord) {
-        const int32_t x = a.perm[j.first + i];
-        a.perm[j.first + i] = a.perm[j.first + c1];
-        a.perm[j.first + c1] = x;
-        c1++;
-      }
-    }
-    if (c1 == 0 || c1 == j.num) c1 = j.num / 2;
-    st[sp++] = BuildJob{fc + 1, j.first + c1, j.num - c1, j.depth + 1, j.rights + 1, {0, 0, 0}};
-    st[sp++] = BuildJob{fc, j.first, c1, j.depth + 1, j.rights, {0, 0, 0}};
+  if (t >= njobs) return;
+  const BuildJob j = a.jobs[2][t];
+  double S1[3];
+  fit_small(a, j, S1);
+  GpuNode& nd = a.nodes[j.bn];
+  atomicMax(&a.counts[3], j.depth);
+  if (j.num == 1) {
+    nd.first_child = -(j.first + 1);
+    return;
   }
-  atomicMax(&a.counts[2], maxd);
+  const int fc = dfs_first_child(j.depth, j.first, j.rights);
+  nd.first_child = fc;
+  const double axis[3] = {nd.R[0], nd.R[3], nd.R[6]};
+  const double coord = split_plane(S1, j.num, axis);
+  int c1 = 0;
+  for (int i = 0; i < j.num; i++) {
+    if (split_coord(tri_at(a, j.first + i), axis) <= coord) {
+      const int32_t x = a.perm[j.first + i];
+      a.perm[j.first + i] = a.perm[j.first + c1];
+      a.perm[j.first + c1] = x;
+      c1++;
+    }
+  }
+  if (c1 == 0 || c1 == j.num) c1 = j.num / 2;
+  push_child(a, BuildJob{fc, j.first, c1, j.depth + 1, j.rights, {0, 0, 0}});
+  push_child(a, BuildJob{fc + 1, j.first + c1, j.num - c1, j.depth + 1, j.rights + 1, {0, 0, 0}});
 }
 
 // ---- parent-relative transforms, sizes and the reordered triangles --------
@@ -501,17 +506,16 @@ hipError_t build_bvh_device(const double* xyz9, const int32_t* ids, const int32_
   *nodes_dev = nullptr;
   *tris_dev = nullptr;
   const int64_t nn = 2 * n - 1;
-  Dev src, dids, dvidx, perm, perm_tmp, tape, cand, jobs0, jobs1, small, counts, mRT, parent,
-      tris, ids_out, vidx_out;
+  Dev src, dids, dvidx, perm, perm_tmp, tape, cand, counts, mRT, parent, tris, ids_out, vidx_out;
+  Dev lists[2][3];   // ping-pong job lists of the three node classes
   GpuNode* nodes = nullptr;
   BVH_TRY(src.alloc((size_t)n * 9 * sizeof(double)));
   BVH_TRY(perm.alloc((size_t)n * 4));
   BVH_TRY(perm_tmp.alloc((size_t)n * 4));
   BVH_TRY(tape.alloc((size_t)n * 8));
   BVH_TRY(cand.alloc((size_t)n * 12));
-  BVH_TRY(jobs0.alloc((size_t)n * sizeof(BuildJob)));
-  BVH_TRY(jobs1.alloc((size_t)n * sizeof(BuildJob)));
-  BVH_TRY(small.alloc((size_t)n * sizeof(BuildJob)));
+  for (int l = 0; l < 2; l++)
+    for (int c = 0; c < 3; c++) BVH_TRY(lists[l][c].alloc((size_t)n * sizeof(BuildJob)));
   BVH_TRY(counts.alloc(4 * sizeof(int32_t)));
   BVH_TRY(hipMalloc((void**)&nodes, (size_t)nn * sizeof(GpuNode)));
   Dev nodes_guard;
@@ -537,42 +541,32 @@ hipError_t build_bvh_device(const double* xyz9, const int32_t* ids, const int32_
   a.tape = (int32_t*)tape.p;
   a.cand = (int32_t*)cand.p;
   a.nodes = nodes;
-  a.small = (BuildJob*)small.p;
   a.counts = (int32_t*)counts.p;
   const BuildJob root{0, 0, (int32_t)n, 0, 0, {0, 0, 0}};
   int32_t host_counts[4] = {0, 0, 0, 0};
-  int64_t nlarge = 0;
   BVH_TRY(hipStreamSynchronize(s));
-  if (n > kSmallNode) {
-    BVH_TRY(hipMemcpy(jobs0.p, &root, sizeof(root), hipMemcpyHostToDevice));
-    nlarge = 1;
-  } else {
-    BVH_TRY(hipMemcpy(small.p, &root, sizeof(root), hipMemcpyHostToDevice));
-    host_counts[1] = 1;
-    BVH_TRY(hipMemcpy(a.counts, host_counts, sizeof(host_counts), hipMemcpyHostToDevice));
-  }
-  BuildJob* cur = (BuildJob*)jobs0.p;
-  BuildJob* nxt = (BuildJob*)jobs1.p;
-  while (nlarge > 0) {
-    BVH_TRY(hipMemsetAsync(a.counts, 0, sizeof(int32_t), s));
-    a.jobs = cur;
-    a.next_large = nxt;
-    a.njobs = (int32_t)nlarge;
-    hipLaunchKernelGGL(build_large, dim3((unsigned)nlarge), dim3(kBlock), 0, s, a);
+  const int rc = node_class((int32_t)n);
+  BVH_TRY(hipMemcpy(lists[0][rc].p, &root, sizeof(root), hipMemcpyHostToDevice));
+  int64_t cnt[3] = {0, 0, 0};
+  cnt[rc] = 1;
+  for (int level = 0; cnt[0] + cnt[1] + cnt[2] > 0; level++) {
+    const int cur = level & 1, nxt = cur ^ 1;
+    for (int c = 0; c < 3; c++) {
+      a.jobs[c] = (const BuildJob*)lists[cur][c].p;
+      a.next[c] = (BuildJob*)lists[nxt][c].p;
+    }
+    BVH_TRY(hipMemsetAsync(a.counts, 0, 3 * sizeof(int32_t), s));
+    if (cnt[0] > 0)
+      hipLaunchKernelGGL(build_node<256>, dim3((unsigned)cnt[0]), dim3(256), 0, s, a, 0);
+    if (cnt[1] > 0)
+      hipLaunchKernelGGL(build_node<64>, dim3((unsigned)cnt[1]), dim3(64), 0, s, a, 1);
+    if (cnt[2] > 0)
+      hipLaunchKernelGGL(build_thread, dim3(grid_of(cnt[2], 64)), dim3(64), 0, s, a,
+                         (int32_t)cnt[2]);
     BVH_TRY(hipGetLastError());
     BVH_TRY(hipMemcpyAsync(host_counts, a.counts, sizeof(host_counts), hipMemcpyDeviceToHost, s));
     BVH_TRY(hipStreamSynchronize(s));
-    nlarge = host_counts[0];
-    BuildJob* t = cur;
-    cur = nxt;
-    nxt = t;
-  }
-  BVH_TRY(hipMemcpyAsync(host_counts, a.counts, sizeof(host_counts), hipMemcpyDeviceToHost, s));
-  BVH_TRY(hipStreamSynchronize(s));
-  const int32_t nsmall = host_counts[1];
-  if (nsmall > 0) {
-    hipLaunchKernelGGL(build_small, dim3(grid_of(nsmall, 64)), dim3(64), 0, s, a, nsmall);
-    BVH_TRY(hipGetLastError());
+    for (int c = 0; c < 3; c++) cnt[c] = host_counts[c];
   }
   BVH_TRY(mRT.alloc((size_t)nn * 12 * sizeof(double)));
   BVH_TRY(parent.alloc((size_t)nn * 4));
@@ -602,7 +596,7 @@ hipError_t build_bvh_device(const double* xyz9, const int32_t* ids, const int32_
                          s));
   BVH_TRY(hipMemcpyAsync(host_counts, a.counts, sizeof(host_counts), hipMemcpyDeviceToHost, s));
   BVH_TRY(hipStreamSynchronize(s));
-  out->max_depth = host_counts[2];
+  out->max_depth = host_counts[3];
   // the mesh keeps the device copies
   *nodes_dev = nodes;
   nodes_guard.p = nullptr;

@@ -154,6 +154,7 @@ int pick_cap(const std::vector<MeshDev>& tab, const int32_t* pairs, int64_t n, i
   }
   if (need <= 48) return 48;
   if (need <= 96) return 96;
+  if (need <= 192) return 192;
   return -1;
 }
 
@@ -239,19 +240,18 @@ void params_to_args(KArgs& a, const c2a_params* p) {
 
 // The autotuned per-size-bucket knobs (tuning_table.h, generated from
 // c2a_amd/tuning.json by the autotuner): the template depth is the deepest
-// bucket's (5 for the learned order, its one instantiated depth), and the
-// replay arena is sized for the largest log among the buckets the batch's
-// queries use (an explicit replay_log >= 0 or ext_levels >= 0 overrides every
-// bucket's).  The extension levels are tuned for the plain order only: under
-// the learned order, and when the batch logs training samples (which are
-// drawn from the replayed records), they stay at 0 unless given.
+// among the buckets the batch's queries use (5 for the learned order, its one
+// instantiated depth), and the replay arena is sized for their largest log
+// (an explicit replay_log >= 0 or ext_levels >= 0 overrides every bucket's).
+// The extension levels are tuned for the plain order only: under the learned
+// order, and when the batch logs training samples (which are drawn from the
+// replayed records), they stay at 0 unless given.
 void apply_tuning(c2a_batch* b, const std::vector<int32_t>& pairs, int replay_log,
                   int ext_levels) {
   if (kTuneRows <= 0) return;
   const int rows = std::min(kTuneRows, C2A_TUNE_MAX);
   KArgs& t = b->tune;
   t.ntune = rows;
-  int kmax = 1;
   for (int r = 0; r < rows; r++) {
     t.tune_ntri[r] = kTune[r][0];
     t.tune_depth[r] = std::min(6, std::max(1, kTune[r][1]));
@@ -261,18 +261,18 @@ void apply_tuning(c2a_batch* b, const std::vector<int32_t>& pairs, int replay_lo
                                     : ((b->scorer || b->sample_cap > 0)
                                            ? 0
                                            : std::min(8, std::max(0, kTune[r][4])));
-    kmax = std::max(kmax, t.tune_depth[r]);
   }
-  b->K = b->scorer ? 5 : kmax;
-  if (replay_log >= 0) return;
-  int logmax = 0;
+  // the kernel's depth and arena are sized for the buckets the batch uses
+  int kmax = 1, logmax = 0;
   for (size_t i = 0; i + 1 < pairs.size(); i += 2) {
     const int big = std::max(b->mtab[(size_t)pairs[i]].ntri, b->mtab[(size_t)pairs[i + 1]].ntri);
     int r = 0;
     while (r + 1 < rows && big > t.tune_ntri[r]) r++;
+    kmax = std::max(kmax, t.tune_depth[r]);
     logmax = std::max(logmax, t.tune_log[r]);
   }
-  b->logcap = logmax;
+  b->K = b->scorer ? 5 : kmax;
+  if (replay_log < 0) b->logcap = logmax;
 }
 
 int fill_args(c2a_batch* b, const c2a_params* p) {
@@ -416,8 +416,9 @@ int c2a_ccd_batch_noalloc(c2a_mesh* const* meshes, int32_t nmesh, const double* 
   }
   // any two meshes of the table may meet: stack capacity from the deepest
   const int need = 2 * maxd + 2;
-  const int CAP = need <= 48 ? 48 : (need <= 96 ? 96 : -1);
+  const int CAP = need <= 48 ? 48 : (need <= 96 ? 96 : (need <= 192 ? 192 : -1));
   if (CAP < 0) return fail(C2A_ERR_UNSUPPORTED, "hierarchies too deep for the traversal stack");
+  if (CAP == 192) K = 5;   // the deepest stack's one depth (no result depends on it)
   const int logcap = p->replay_log == -1 ? 1024 : p->replay_log;
   if (logcap < 0) return fail(C2A_ERR_BAD_ARG, "replay_log must be -1 (1024) or >= 0");
   const size_t room = kScratchBytes - kScratchArena;
@@ -757,6 +758,8 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     // spec_depth 0: the autotuned knobs of each query's size bucket
     if (pp->spec_depth == 0) apply_tuning(b, pairs, pp->replay_log, pp->ext_levels);
   }
+  // the deepest stack is instantiated at one depth; the depth changes no result
+  if (b->CAP == 192) b->K = 5;
   b->blocks = grid_blocks(b->K, b->CAP, b->NW, nq, b->scorer != nullptr);
   const bool feats = b->scorer || b->sample_cap > 0;
   if (b->logcap > 0 && nq > 0) {

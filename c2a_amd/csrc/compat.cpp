@@ -128,6 +128,13 @@ void Matrix3x3::Set_Value(const Quaternion& q) {
   c2a::matrix_from_quat(qq, m_);
 }
 
+// the engine's matrix-to-quaternion conversion (the motion constructor's)
+Quaternion Matrix3x3::Quaternion_() {
+  double q[4];
+  c2a::quat_from_matrix(m_, q);
+  return Quaternion(q[0], q[1], q[2], q[3]);
+}
+
 // ---------------------------------------------------------------- C2A_Model
 C2A_Model::C2A_Model()
     : tris(nullptr), num_tris(0), last_tri(nullptr), build_state(0), mesh_(nullptr) {}

@@ -7,6 +7,7 @@
 // output: int32 n, n*9 doubles (triangles in hierarchy order), n int32 ids,
 //         int32 last_tri index, then per motion: axis (3), angle (1) and per
 //         time the pose (12) and (w, x, y, z, t) from integrate (7)
+#include <math.h>
 #include <stdio.h>
 
 #include <vector>
@@ -30,6 +31,48 @@ int main(int argc, char** argv) {
   std::vector<double> ts((size_t)nt);
   if (fread(ts.data(), 8, ts.size(), f) != ts.size()) return 3;
   fclose(f);
+  {
+    // LinearMath's operators (LinearMath.h:453-493 and their definitions)
+    auto near = [](double a, double b) { return fabs(a - b) <= 1e-12; };
+    Coord3D a(1, 2, 3), b(-2, 0.5, 4);
+    if (a * b != 1 * -2 + 2 * 0.5 + 3 * 4.0) return 7;
+    const Coord3D c = a % b;
+    if (!near(c * a, 0) || !near(c * b, 0) || !((a + b) - b == a) || !(-a == Coord3D(-1, -2, -3)))
+      return 7;
+    Matrix3x3 M;
+    M.Set_Value(0.3, -0.2, 0.7);   // Rz (Ry Rx)
+    const Matrix3x3 I1 = M % M, I2 = M / M, I3 = M.Inverse() * M;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        const double e = i == j ? 1.0 : 0.0;
+        if (!near(I1[i][j], e) || !near(I2[i][j], e) || !near(I3[i][j], e)) return 7;
+      }
+    const Coord3D v = M * (M % a);
+    if (!near(v[0], 1) || !near(v[1], 2) || !near(v[2], 3)) return 7;
+    Matrix3x3 Mt = M;
+    Mt.Transpose();
+    if (Mt[0][1] != M[1][0] || Mt[2][0] != M[0][2]) return 7;
+    Quaternion q(Coord3D(1, 1, 0), 0.8);   // 0.8 rad about (1, 1, 0)
+    Matrix3x3 Mq;
+    Mq.Set_Value(q);
+    Quaternion q2 = Mq.Quaternion_();
+    const double sgn = q2.W() * q.W() < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < 4; i++)
+      if (!near(sgn * q2[i], q[i])) return 7;
+    Quaternion qc = q;
+    qc.Invert();
+    const Quaternion e = q % qc;   // q q* = |q|^2
+    if (!near(e.W(), q * q) || !near(e.X(), 0) || !near(e.Y(), 0) || !near(e.Z(), 0)) return 7;
+    Transform X(M, Coord3D(0.5, -1, 2)), Xi;
+    Xi.Invert(X);
+    const Coord3D p = Xi * (X * a), r = X & a;
+    if (!near(p[0], 1) || !near(p[1], 2) || !near(p[2], 3)) return 7;
+    const Coord3D r2 = M * a;
+    if (!(r == r2)) return 7;
+    Coord3D w = a;
+    w ^= X;
+    if (!(w == X * a)) return 7;
+  }
   {
     // ContactF as C2A.h:116-153 / ListTri.h:43-61 declare it
     int fa[3] = {1, 2, 3}, fb[3] = {4, -1, -1};

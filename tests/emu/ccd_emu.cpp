@@ -217,8 +217,10 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
     a.sample_count = &g_scount;
   }
   a.feats = (a.scorer.on || a.sample_cap > 0) ? 1 : 0;
-  // stack capacity as the library picks it
-  const bool small = b1.max_depth + b2.max_depth + 2 <= 48;
+  // stack capacity as the library picks it (the deepest one at depth 5)
+  const int need = b1.max_depth + b2.max_depth + 2;
+  const bool small = need <= 48;
+  if (need > 96) return need <= 192 ? run<5, 192>(a) : 2;
   switch (K) {
     case 1: return small ? run<1, 48>(a) : run<1, 96>(a);
     case 2: return small ? run<2, 48>(a) : run<2, 96>(a);

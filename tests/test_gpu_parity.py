@@ -97,3 +97,20 @@ def test_knot_batch_4096_full_bit_exact(knot_pair):
     assert_same(ref, got, "knot 4096")
     frac = float((got["collisionfree"] == 0).mean())
     assert 0.25 < frac < 0.35, frac
+
+
+def test_deep_hierarchy_pair_bit_exact():
+    """Hierarchy pairs deeper than 96 levels run on the 192-entry stack."""
+    from tests.test_kernel_logic_cpu import chain_queries, factorial_chain
+    t1, t2 = factorial_chain(100, 1), factorial_chain(100, 2, mirror=True)
+    meshes = [G.Mesh(t1), G.Mesh(t2)]
+    assert meshes[0].max_depth + meshes[1].max_depth + 2 > 96
+    nq = 64
+    P = chain_queries(nq, 808)
+    seeds = np.full((nq, 2), 99, np.int32)
+    print("deep hierarchy pair nq=%d" % nq, flush=True)
+    got = G.ccd_batch(meshes, G.make_queries(*P, mesh_pairs=np.tile([0, 1], (nq, 1)), seeds=seeds),
+                      G.default_params(tol_d=1e-6, tol_t=1e-6))
+    ref = O.batch_toc(O.OModel(t1), O.OModel(t2), *P, 1e-6, 1e-6, seeds=seeds,
+                      with_contacts=True, nthreads=8)
+    assert_same(ref, got, "deep")

@@ -163,3 +163,54 @@ def test_learned_order_is_conservative_on_cpu(emu_learned):
         return (r["collisionfree"] == 0) & (r["toc"] > 0) & (r["distance"] == 0)
     assert not (over(learned) & ~over(plain)).any()
     assert not np.array_equal(plain["bv_tests"], learned["bv_tests"])
+
+
+def factorial_chain(n, seed, mirror=False):
+    """Triangles along x at k!/(n-1)!, k = 0..n-1, each scaled with its
+    position: every mean split peels off the farthest one, so the hierarchy
+    is a chain about 0.7 n levels deep (the tiniest ones merge).  mirror puts
+    it on the negative x axis."""
+    import math
+    rng = np.random.default_rng(seed)
+    top = float(math.factorial(n - 1))
+    out = []
+    for i in range(n):
+        x = math.factorial(i) / top
+        t = np.array([[0, 0, 0], [0.3, 0, 0], [0, 0.3, 0.1]]) * x * (1 + 0.1 * rng.random())
+        t = t + np.array([x, 0.2 * x * rng.random(), 0.2 * x * rng.random()])
+        if mirror:
+            t[:, 0] *= -1
+        out.append(t.ravel())
+    return np.array(out)
+
+
+def chain_queries(nq, seed):
+    """The mirrored chain moving from a small gap across the first chain's
+    tip while turning slightly about z."""
+    rng = np.random.default_rng(seed)
+    P = [np.zeros((nq, 12)) for _ in range(4)]
+    for i in range(nq):
+        for k in range(4):
+            P[k][i, :9] = np.eye(3).ravel()
+        a = rng.uniform(1e-3, 3e-2)
+        c, s = np.cos(a), np.sin(a)
+        P[3][i, :9] = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]]).ravel()
+        g = rng.uniform(1e-3, 5e-3)
+        P[2][i, 9:] = [-g, rng.uniform(-1e-3, 1e-3), 0]
+        P[3][i, 9:] = [2 * g, P[2][i, 10], 0]
+    return P
+
+
+def test_kernel_logic_deep_hierarchies(emu):
+    """Two hierarchies about 70 levels deep: the pair runs on the 192-entry
+    stack (the walks themselves stay shallow: the error bounds prune the
+    chains' tips, which all lie within the gap)."""
+    t1, t2 = factorial_chain(100, 1), factorial_chain(100, 2, mirror=True)
+    assert G.Mesh(t1).max_depth + G.Mesh(t2).max_depth + 2 > 96
+    nq = 12
+    P = chain_queries(nq, 808)
+    seeds = np.full((nq, 2), 99, np.int32)
+    m1, m2 = O.OModel(t1), O.OModel(t2)
+    ref = O.batch_toc(m1, m2, *P, 1e-6, 1e-6, seeds=seeds, with_contacts=True)
+    got = run_emu(emu, t1, t2, P, G.default_params(tol_d=1e-6, tol_t=1e-6), 5, seeds=seeds)
+    same(ref, got)
