@@ -81,19 +81,27 @@ C2A_HD void jacobi3(const double* A, double* V, double* d) {
 }
 
 // Vertex covariance accumulation of one triangle (C2A_Build.cpp:279-335):
-// acc = S1[3], then s00, s11, s22, s01, s02, s12.  S1 is also the split's
+// the nine terms t = S1[3], then s00, s11, s22, s01, s02, s12.  S1 is also the split's
 // vertex sum (C2A_Build.cpp:461-472 sums the same terms in the same order).
-C2A_HD void cov_add(double* acc, const double* p) {
+C2A_HD void cov_terms(const double* p, double* t) {
   const double *p1 = p, *p2 = p + 3, *p3 = p + 6;
-  acc[0] += p1[0] + p2[0] + p3[0];
-  acc[1] += p1[1] + p2[1] + p3[1];
-  acc[2] += p1[2] + p2[2] + p3[2];
-  acc[3] += (p1[0] * p1[0] + p2[0] * p2[0] + p3[0] * p3[0]);
-  acc[4] += (p1[1] * p1[1] + p2[1] * p2[1] + p3[1] * p3[1]);
-  acc[5] += (p1[2] * p1[2] + p2[2] * p2[2] + p3[2] * p3[2]);
-  acc[6] += (p1[0] * p1[1] + p2[0] * p2[1] + p3[0] * p3[1]);
-  acc[7] += (p1[0] * p1[2] + p2[0] * p2[2] + p3[0] * p3[2]);
-  acc[8] += (p1[1] * p1[2] + p2[1] * p2[2] + p3[1] * p3[2]);
+  t[0] = p1[0] + p2[0] + p3[0];
+  t[1] = p1[1] + p2[1] + p3[1];
+  t[2] = p1[2] + p2[2] + p3[2];
+  t[3] = (p1[0] * p1[0] + p2[0] * p2[0] + p3[0] * p3[0]);
+  t[4] = (p1[1] * p1[1] + p2[1] * p2[1] + p3[1] * p3[1]);
+  t[5] = (p1[2] * p1[2] + p2[2] * p2[2] + p3[2] * p3[2]);
+  t[6] = (p1[0] * p1[1] + p2[0] * p2[1] + p3[0] * p3[1]);
+  t[7] = (p1[0] * p1[2] + p2[0] * p2[2] + p3[0] * p3[2]);
+  t[8] = (p1[1] * p1[2] + p2[1] * p2[2] + p3[1] * p3[2]);
+}
+
+// acc[k] += t[k]: the reference adds each right-hand side to its sum, so a
+// sum's terms can be computed anywhere as long as they are added in order.
+C2A_HD void cov_add(double* acc, const double* p) {
+  double t[9];
+  cov_terms(p, t);
+  for (int k = 0; k < 9; k++) acc[k] += t[k];
 }
 
 C2A_HD void cov_finish(const double* acc, int n, double* M) {

@@ -5,13 +5,15 @@
 // Level by level, every node of a level at once, in three sizes: a
 // 256-thread workgroup per node above kWaveMax triangles, one wavefront per
 // node down to kThreadMax, one thread per node below.  A workgroup (or
-// wavefront) stages the node's triangles through LDS and one lane runs the
-// covariance sums in triangle order (sums are order-dependent; they are the
-// only sequential part); every minimum, maximum and arg-extreme of the fit is
-// reduced across the lanes (exact in any order); the corner pass runs
-// sequentially over the few points that can grow a corner; and the in-place
-// partition of the split is reproduced in parallel from prefix counts (see
-// the split below).  A one-thread node runs the host's sequential code.
+// wavefront) stages the node's per-triangle covariance terms through LDS and
+// nine lanes add them to the nine sums in triangle order (sums are
+// order-dependent: one dependent add per triangle and sum is the only
+// sequential part); every minimum, maximum and arg-extreme of the fit is
+// reduced across the lanes (exact in any order); the corner pass evaluates 64
+// points at a time against the current rectangle and advances to the first
+// one that grows it; and the in-place partition of the split is reproduced in
+// parallel from prefix counts (see the split below).  A one-thread node runs
+// the host's sequential code.
 // Node numbers follow the host's depth-first order through dfs_first_child.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -158,25 +160,33 @@ __global__ void __launch_bounds__(B) build_node(BuildArgs a, int cls) {
   const int32_t first = job.first, num = job.num;
   const int np = 3 * num;
 
-  // covariance sums, in triangle order (C2A_Build.cpp:279-335)
-  if (tid == 0)
-    for (int k = 0; k < 9; k++) acc[k] = 0.0;
+  // covariance sums, in triangle order (C2A_Build.cpp:279-335): every
+  // thread computes its triangle's nine terms into LDS, then lane k < 9 adds
+  // term k of each triangle, in order, to sum k (one dependent add per
+  // triangle and sum)
+  double run = 0.0;
   for (int base = 0; base < num; base += B) {
     const int i = base + tid;
-    if (i < num) {
-      const double* t = tri_at(a, first + i);
-      for (int k = 0; k < 9; k++) chunk[tid * 9 + k] = t[k];
-    }
+    if (i < num) cov_terms(tri_at(a, first + i), chunk + 9 * tid);
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 9) {
       const int m = num - base < B ? num - base : B;
-      double s[9];
-      for (int k = 0; k < 9; k++) s[k] = acc[k];
-      for (int j = 0; j < m; j++) cov_add(s, chunk + 9 * j);
-      for (int k = 0; k < 9; k++) acc[k] = s[k];
+      double sk = run;
+      int j = 0;
+      for (; j + 8 <= m; j += 8) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = chunk[9 * (j + u) + tid];
+#pragma unroll
+        for (int u = 0; u < 8; u++) sk += t[u];
+      }
+      for (; j < m; j++) sk += chunk[9 * j + tid];
+      run = sk;
     }
     __syncthreads();
   }
+  if (tid < 9) acc[tid] = run;
+  __syncthreads();
   if (tid == 0) {
     double C[9], E[9], sv[3], Rl[9], s9[9];
     for (int k = 0; k < 9; k++) s9[k] = acc[k];
@@ -251,12 +261,47 @@ __global__ void __launch_bounds__(B) build_node(BuildArgs a, int cls) {
     if (f) cand[ncand + at] = i;
     ncand += tot;
   }
+  // The host grows the rectangle point by point (C2A_BV.cpp:542-606); a
+  // point changes it only when it lies outside the current rounded corner.
+  // Wavefront 0 evaluates 64 candidates at once against the current
+  // rectangle: the points before the first one that changes it are no-ops
+  // in sequence too, and that one's result is the next rectangle.  Each
+  // chunk's projected points are staged in LDS by the whole workgroup.
   __syncthreads();
-  if (tid == 0) {
-    for (int c = 0; c < ncand; c++) {
-      proj(a, Rr, first, cand[c], P);
-      corner_grow(P[0], P[1], P[2], cz, radsqr, minx, maxx, miny, maxy);
+  double* cp = chunk;
+  for (int base = 0; base < ncand; base += B) {
+    const int m = ncand - base < B ? ncand - base : B;
+    if (tid < m) proj(a, Rr, first, cand[base + tid], cp + 3 * tid);
+    __syncthreads();
+    if (tid < 64) {
+      int pos = 0;
+      while (pos < m) {
+        const int j = pos + tid;
+        double x0 = minx, x1 = maxx, y0 = miny, y1 = maxy;
+        bool changed = false;
+        if (j < m) {
+          corner_grow(cp[3 * j], cp[3 * j + 1], cp[3 * j + 2], cz, radsqr, x0, x1, y0, y1);
+          changed = __double_as_longlong(x0) != __double_as_longlong(minx) ||
+                    __double_as_longlong(x1) != __double_as_longlong(maxx) ||
+                    __double_as_longlong(y0) != __double_as_longlong(miny) ||
+                    __double_as_longlong(y1) != __double_as_longlong(maxy);
+        }
+        const unsigned long long hit = __ballot(changed);
+        if (hit == 0) {
+          pos += 64;
+          continue;
+        }
+        const int f = __ffsll(hit) - 1;
+        minx = __shfl(x0, f);
+        maxx = __shfl(x1, f);
+        miny = __shfl(y0, f);
+        maxy = __shfl(y1, f);
+        pos += f + 1;
+      }
     }
+    __syncthreads();
+  }
+  if (tid == 0) {
     GpuNode& nd = a.nodes[job.bn];
     for (int k = 0; k < 9; k++) nd.R[k] = Rr[k];
     nd.angR = angR;
