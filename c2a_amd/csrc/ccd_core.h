@@ -205,6 +205,7 @@ C2A_HD Cur cur_from_hot(const HotRec& h0, int rec, int nn, int lvl) {
 }
 
 constexpr int WIN = 64;  // replayed records staged in LDS for the walk
+constexpr int MR_MAX = 16; // misses one round can cover
 constexpr int LBUF = 64; // log entries staged in LDS before they go to the arena
 
 struct Slot {            // speculative pair values, heap-indexed below X
@@ -250,6 +251,13 @@ struct Wave {
   int32_t rpb;               // ... as pairs [rpb, rpb + 2^K - 1)
   int32_t lh[CAP + 2];       // replay: expansions per level, then fill cursors
   int32_t ls[CAP + 3];       // replay: first sorted expansion of each level
+  // Misses ahead in the scan's window share a round: each gets a heap of
+  // `mpage` slots (mpage = 0: one heap below X).
+  int32_t mpage;             // slots per heap of the current round
+  int32_t mrgen;             // generation of the round below the gathered misses
+  int32_t nmr;               // misses gathered (then: covered by that round)
+  int32_t mrrec[MR_MAX];     // their replayed record numbers, in scan order
+  int32_t mrb[MR_MAX][2];    // and node ids
 };
 
 // A mesh-table entry held in LDS, made wavefront-uniform so that node and
@@ -599,16 +607,15 @@ C2A_HD bool ca_on_tris(const Motion& mo1, const double* r1, const double* A, con
 
 // Heap slot h of the depth-K subtree below X: the pair reached by the path
 // bits of h+1, its transform chained from X, and its values.
-template <int K, int CAP>
-C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
-                              bool contact, Slot& s) {
+template <int K, int CAP, class RT>
+C2A_HD void compute_slot_below(Wave<K, CAP>& g, const RT* xR, const RT* xT, int b1, int b2, int h,
+                               const MeshDev& M1, const MeshDev& M2, bool contact, Slot& s) {
   node_ptr N1 = nodes_of(M1);
   node_ptr N2 = nodes_of(M2);
   const int lvl = 31 - clz32((unsigned)(h + 1));
   double R[9], T[3];
-  for (int i = 0; i < 9; i++) R[i] = g.X.R[i];
-  for (int i = 0; i < 3; i++) T[i] = g.X.T[i];
-  int b1 = g.X.b1, b2 = g.X.b2;
+  for (int i = 0; i < 9; i++) R[i] = xR[i];
+  for (int i = 0; i < 3; i++) T[i] = xT[i];
   NodeHead h1 = head_of(N1 + b1), h2 = head_of(N2 + b2);
   for (int lv = 0; lv < lvl; lv++) {
     const int bit = ((h + 1) >> (lvl - 1 - lv)) & 1;
@@ -641,6 +648,12 @@ C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const M
   if (leafp)
     leaf_values_tris(s, N1 + b1, N2 + b2, ta, tb, t1, t2, g.Rrel, g.Trel, g.r1, g.tt1, g.mo[0],
                      g.mo[1], contact);
+}
+
+template <int K, int CAP>
+C2A_HD void compute_slot_into(Wave<K, CAP>& g, int h, const MeshDev& M1, const MeshDev& M2,
+                              bool contact, Slot& s) {
+  compute_slot_below(g, g.X.R, g.X.T, g.X.b1, g.X.b2, h, M1, M2, contact, s);
 }
 
 template <int K, int CAP>
@@ -705,6 +718,20 @@ C2A_HD void round_lane(Wave<K, CAP>& g, const Leader& L, int lane, int nl, const
     for (int h = lane + 1; h < Shape<K>::NSLOT; h += nl) compute_slot_arena(g, h, M1, M2, av);
     return;
   }
+  const int mp = g.mpage;
+  if (mp) {
+    // heap r (slots [r * mp, r * mp + mp - 1)) below gathered miss r, whose
+    // transform is its replayed record's
+    const ArenaView av = arena_view(L.ab, a.logcap);
+    const int nr = g.nmr;
+    for (int h = lane; h < nr * mp; h += nl) {
+      const int r = h / mp, l = h - r * mp;
+      if (l == 0 || l == mp - 1) continue;   // the miss itself; padding
+      const double C2A_GLOBAL* xf = gp(av.xf) + 12 * (int64_t)g.mrrec[r];
+      compute_slot_below(g, xf, xf + 9, g.mrb[r][0], g.mrb[r][1], l, M1, M2, contact, g.slot[h]);
+    }
+    return;
+  }
   const int nslot = (2 << g.rdepth) - 1;   // levels 0..rdepth of the heap
   for (int h = lane + 1; h < nslot; h += nl) compute_slot(g, h, M1, M2, contact);
 }
@@ -716,6 +743,7 @@ C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, 
   if (gen >= 0x7fffff) {   // generation wrap: forget every cached link
     for (int i = lane; i < L.sp; i += nl) g.stack[i].sg = 0;
     gen = 1;
+    if (lane == 0) g.mrgen = -1;
   }
   const int flags = e.flags;
   const int ar = e.ar;
@@ -737,14 +765,31 @@ C2A_HD void request_round(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, 
   constexpr int NPAIR = (1 << K) - 1;
   const bool to_arena = a.arena_rounds && !a.scorer.on && a.logcap > 0 && !g.contact &&
                         (flags & 3) != 3 && L.napp + NPAIR <= 2 * a.logcap;
+  // a miss the scan found with more misses after it in its window: one
+  // shallow heap below each (the same slots a round below each would fill)
+  const int mpage = 2 << L.kmiss;
+  const bool multi = !to_arena && (flags & 8) && (flags & 3) != 3 && L.kmiss > 0 &&
+                     L.kmiss < L.kdepth && 2 * mpage <= Shape<K>::NSLOT + 1 && g.nmr > 1 &&
+                     ar == g.mrrec[0];
+  lane_sync();
   if (lane == 0) {
-    if (to_arena) {
+    if (multi) {
+      g.rarena = 0;
+      e.sg = sg;               // heap 0, slot 0: the miss being walked
+      g.gen = gen;
+      g.mrgen = gen;
+      g.mpage = mpage;
+      g.rdepth = L.kmiss;
+      const int cap = (Shape<K>::NSLOT + 1) / mpage;
+      if (g.nmr > cap) g.nmr = cap;
+    } else if (to_arena) {
       g.rarena = 1;
       g.rpb = L.napp;
       e.cp = L.napp;
       e.flags = flags | 8;
     } else {
       g.rarena = 0;
+      g.mpage = 0;
       e.sg = sg;
       Slot& s0 = g.slot[0];
       s0.b1 = g.X.b1;
@@ -793,6 +838,9 @@ C2A_HD void start_traversal(Wave<K, CAP>& g, Leader& L, const MeshDev& M1, const
     for (int i = 0; i < 9; i++) { g.Rrel[i] = Rrel[i]; g.r1[i] = R1[i]; }
     for (int i = 0; i < 3; i++) { g.Trel[i] = Trel[i]; g.tt1[i] = T1[i]; }
     g.gen = 0;
+    g.mpage = 0;
+    g.mrgen = -1;
+    g.nmr = 0;
     Entry& e = g.stack[0];
     for (int i = 0; i < 9; i++) e.R[i] = R[i];
     for (int i = 0; i < 3; i++) e.T[i] = T[i];
@@ -1386,13 +1434,38 @@ C2A_HD int scan(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
         lane_sync();
         continue;
       }
-      // a pair the previous step did not expand: sequential walk below it
+      // a pair the previous step did not expand: sequential walk below it.
+      // If the last round was the one below an earlier miss of this window
+      // and covered this one too, its heap is still in the slots; otherwise
+      // this miss and the misses after it in the window are gathered for
+      // the next round.
+      int csg = 0;
+      const int mgen = g.mrgen, nmr = g.nmr, mpg = g.mpage;
+      if (mgen == g.gen)
+        for (int r = 1; r < nmr; r++)
+          if (g.mrrec[r] == srec) csg = (mgen << 8) | (r * mpg + 1);
+      lane_sync();
+      if (csg == 0) {
+        const bool gather = miss && j >= stop;
+        const uint64_t mm = wave_ballot(gather);
+        const int r = lanes_below(mm, j);
+        if (gather && r < MR_MAX && lane < 64) {
+          g.mrrec[r] = E.rec;
+          g.mrb[r][0] = E.b1;
+          g.mrb[r][1] = E.b2;
+        }
+        if (lane == 0) {
+          const int cnt = (int)__builtin_popcountll(mm);
+          g.nmr = cnt < MR_MAX ? cnt : MR_MAX;
+          g.mrgen = -1;
+        }
+      }
       Cur c;
       c.d = sd;
       c.mt = smt;
       c.b1 = sb1;
       c.b2 = sb2;
-      c.sg = 0;
+      c.sg = csg;
       c.flags = (sinfo & 3) | 8;
       c.dTri = sdt;
       c.mleaf = sml;
@@ -1497,6 +1570,7 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
   const bool contact = cmode != 0;
   const bool learned = a.scorer.on && !contact;
   const int gen = uni(g.gen);
+  const int mp = uni(g.mpage);
   const ArenaView av = arena_view(L.ab, a.logcap);
   const int nrec_max = arena_records(a.logcap);
   Cur c;
@@ -1586,7 +1660,8 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
       continue;
     }
     const bool from_arena = (flags & 8) && c.cp >= 0;
-    const int lvl = cached ? 31 - clz32((unsigned)(s + 1)) : K;
+    const int loc = mp ? (s & (mp - 1)) : s;   // index within its heap
+    const int lvl = cached ? 31 - clz32((unsigned)(loc + 1)) : K;
     if (!from_arena && (!cached || lvl >= uni(g.rdepth))) {
       C2A_EMU_COUNT((flags & 8) ? 1 : (cached ? 2 : 3), L);
       push_back(g, L, c, where, csl, lane, nl);
@@ -1636,7 +1711,7 @@ C2A_HD bool walk(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
       have = true;
       continue;
     }
-    const int sa = 2 * s + 1, sc = 2 * s + 2;
+    const int sa = s + loc + 1, sc = sa + 1;   // children in the same heap
     // pairs first reached in this step keep the closer-child-first order:
     // scoring a round's slots costs more than the better order saves
     const double dA = unid(g.slot[sa].d), dC = unid(g.slot[sc].d);

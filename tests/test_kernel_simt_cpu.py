@@ -12,7 +12,9 @@ primitives, or a lane that finishes while others wait, fail the run.
 Each case is compared field by field with the FP64 oracle, including the
 arena-backed rounds (KArgs.arena_rounds) that the library keeps switched
 off, small replay logs (the log-full paths), the replay's extension, the
-strict mode and the learned order.
+strict mode, the learned order, and the rounds that several misses of a
+scan window share (one lane decides one event per scan pass, so only the
+64-lane run gathers them).
 
 The same tests run under AddressSanitizer by pointing CCD_EMU64 at an ASan
 build of the same sources:
@@ -70,6 +72,9 @@ CASES = [
     dict(K=5, arena=1, replay_log=0),       # no replay at all
     dict(K=5, arena=1, ext_levels=1),
     dict(K=4, arena=0, ext_levels=2),
+    dict(K=5, arena=0, miss_depth=1),       # 16 misses per shared round
+    dict(K=5, arena=0, miss_depth=3),       # 4 per round
+    dict(K=4, arena=0, miss_depth=3),       # 2 per round
 ]
 
 
@@ -114,3 +119,21 @@ def test_simt_learned_order_matches_one_lane(emu64, knot):
             lib.emu_set_scorer(None)
     for f in ("toc", "distance", "mint", "numCA", "bv_tests", "tri_tests", "last_tri"):
         assert np.array_equal(outs[0][f], outs[1][f]), f
+
+
+@pytest.mark.parametrize("miss_depth", [1, 2, 3])
+def test_simt_misses_share_rounds(emu64, knot, miss_depth):
+    """Misses found in one 64-event scan window share a round (a shallow heap
+    below each): bit-exact, and fewer rounds than the one-lane run, whose
+    scan sees one event at a time."""
+    t9, P, ref = knot
+    emu1 = C.CDLL(os.path.join(HERE, "emu", "libccd_emu.so"))
+    rounds = []
+    for lib in (emu1, emu64):
+        lib.emu_rounds.restype = C.c_int64
+        lib.emu_set_arena_rounds(0)
+        got = run_emu(lib, t9, t9, P, G.default_params(tol_d=1e-3, tol_t=1e-4, spec_depth=5,
+                                                        miss_depth=miss_depth), 5)
+        same(ref, got)
+        rounds.append(lib.emu_rounds())
+    assert rounds[1] < 0.95 * rounds[0], rounds
