@@ -40,7 +40,8 @@ class Params(C.Structure):
                 ("waves_per_query", C.c_int32), ("replay_log", C.c_int32), ("order", C.c_int32),
                 ("scorer", C.c_void_p), ("sample_rate", C.c_double), ("sample_cap", C.c_int64),
                 ("miss_depth", C.c_int32), ("ext_levels", C.c_int32),
-                ("translation", C.c_int32), ("strict", C.c_int32)]
+                ("translation", C.c_int32), ("strict", C.c_int32),
+                ("tail_waves", C.c_int32), ("tail_budget", C.c_int64)]
 
 
 class Result(C.Structure):
@@ -105,7 +106,7 @@ EXPORTS = ["c2a_params_default", "c2a_mesh_create", "c2a_mesh_destroy", "c2a_mes
            "c2a_batch_samples", "c2a_ccd_batch_noalloc", "c2a_contact_list",
            "c2a_batch_run_graph", "c2a_distance_queue", "c2a_distance_contact_space",
            "c2a_tolerance", "c2a_collide_pairs", "c2a_mesh_create_device", "c2a_mesh_save",
-           "c2a_mesh_load"]
+           "c2a_mesh_load", "c2a_batch_tail_count"]
 
 
 def lib():
@@ -138,6 +139,7 @@ def lib():
     L.c2a_batch_fetch.argtypes = [vp, vp, vp]
     L.c2a_batch_destroy.argtypes = [vp]
     L.c2a_batch_last_ms.argtypes = [vp, C.POINTER(C.c_float)]
+    L.c2a_batch_tail_count.argtypes = [vp, C.POINTER(C.c_int64)]
     L.c2a_batch_samples.argtypes = [vp, vp, vp, vp, i64, C.POINTER(i64), C.POINTER(i64)]
     L.c2a_scorer_save.argtypes = [vp, C.c_char_p]
     L.c2a_scorer_load.argtypes = [C.c_char_p, C.POINTER(vp)]
@@ -315,6 +317,12 @@ class Batch:
         ms = C.c_float()
         _check(lib().c2a_batch_last_ms(self.handle, C.byref(ms)))
         return ms.value
+
+    def tail_count(self):
+        """Queries the last run handed to the tail pass (0: one launch)."""
+        n = C.c_int64()
+        _check(lib().c2a_batch_tail_count(self.handle, C.byref(n)))
+        return n.value
 
     def samples(self, cap):
         """Training samples of the last run: (features (n, 32) float32, labels (n,),

@@ -99,8 +99,12 @@ struct c2a_mesh {
 struct c2a_batch {
   std::vector<MeshDev> mtab;
   DevBuf d_mesh, d_q, d_out, d_counter, d_arena, d_samples, d_keys, d_scount;
+  DevBuf d_tail, d_tlist, d_tctr;         // tail pass: saved states, hand-overs, counters
   int64_t nq = 0;
   KArgs args{};
+  KArgs args2{};                         // the tail pass's arguments (tail = true)
+  bool tail = false;
+  int blocks2 = 0;
   int K = 5, CAP = 48, NW = 4, blocks = 1;
   int logcap = 0;
   const c2a_scorer* scorer = nullptr;   // learned order
@@ -166,7 +170,8 @@ int pick_shape(const c2a_params* p, int& K, int& NW) {
     return fail(C2A_ERR_BAD_ARG, "order must be C2A_ORDER_PLAIN or C2A_ORDER_LEARNED");
   if (learned && !p->scorer) return fail(C2A_ERR_BAD_ARG, "learned order needs a scorer");
   if (K < 1 || K > 6) return fail(C2A_ERR_BAD_ARG, "spec_depth must be 1..6");
-  if (!kernel_shape_ok(K, 48, NW, false)) return fail(C2A_ERR_BAD_ARG, "waves_per_query must be 0 or 1");
+  if (NW != 1 || !kernel_shape_ok(K, 48, NW, false))
+    return fail(C2A_ERR_BAD_ARG, "waves_per_query must be 0 or 1");
   if (learned && !kernel_shape_ok(K, 48, NW, true))
     return fail(C2A_ERR_BAD_ARG, "learned order runs at spec_depth 0 or 5");
   return C2A_OK;
@@ -309,7 +314,15 @@ int fill_args(c2a_batch* b, const c2a_params* p) {
   return C2A_OK;
 }
 
+// The tail pass's kernel shape.
+constexpr int kTailK = 5;
+constexpr int kTailWaves = 4;
+
 __global__ void zero_counter(int64_t* c) { *c = 0; }
+__global__ void zero_tail(int64_t* c, int32_t* n) {
+  *c = 0;
+  *n = 0;
+}
 
 int run_kernel(const KArgs& a, int K, int CAP, int NW, int blocks, hipStream_t s) {
   // counter cleared by a kernel (graph-capture safe, no memset node)
@@ -317,6 +330,46 @@ int run_kernel(const KArgs& a, int K, int CAP, int NW, int blocks, hipStream_t s
   HIP_TRY(hipGetLastError());
   hipError_t e = launch_ccd(a, K, CAP, NW, blocks, s);
   if (e != hipSuccess) return hip_fail(e, "launch_ccd");
+  return C2A_OK;
+}
+
+// A batch's launches: the CCD kernel, then (tail pass) the second kernel
+// over the queries the first one handed over, four wavefronts per query.
+int run_batch(const c2a_batch* b, hipStream_t s) {
+  if (b->tail) {
+    hipLaunchKernelGGL(zero_tail, dim3(1), dim3(1), 0, s, b->args2.counter, b->args.tail_count);
+    HIP_TRY(hipGetLastError());
+  }
+  int rc = run_kernel(b->args, b->K, b->CAP, b->NW, b->blocks, s);
+  if (rc || !b->tail) return rc;
+  const hipError_t e = launch_ccd(b->args2, kTailK, b->CAP, kTailWaves, b->blocks2, s);
+  if (e != hipSuccess) return hip_fail(e, "launch_ccd (tail pass)");
+  return C2A_OK;
+}
+
+// Sets up the tail pass when the batch can take it: plain order, no sample
+// logging, a stack the four-wavefront shapes cover.
+int setup_tail(c2a_batch* b, const c2a_params* p) {
+  const int64_t budget = p->tail_budget;
+  if (budget <= 0 || b->nq < 2 || b->scorer || b->sample_cap > 0 ||
+      !kernel_shape_ok(kTailK, b->CAP, kTailWaves, false))
+    return C2A_OK;
+  int rc;
+  if ((rc = b->d_tail.alloc((size_t)b->nq * sizeof(TailRec))) ||
+      (rc = b->d_tlist.alloc((size_t)b->nq * sizeof(int32_t))) ||
+      (rc = b->d_tctr.alloc(16)))
+    return rc;
+  KArgs& a = b->args;
+  a.tail_budget = budget;
+  a.tail_rec = (TailRec*)b->d_tail.p;
+  a.tail_list = (int32_t*)b->d_tlist.p;
+  a.tail_count = (int32_t*)((char*)b->d_tctr.p + 8);
+  b->args2 = a;
+  b->args2.tail_pass = 1;
+  b->args2.counter = (int64_t*)b->d_tctr.p;
+  // one replay arena per workgroup: no more workgroups than the first launch
+  b->blocks2 = std::min(grid_blocks(kTailK, b->CAP, kTailWaves, b->nq), b->blocks);
+  b->tail = true;
   return C2A_OK;
 }
 
@@ -471,6 +524,8 @@ void c2a_params_default(c2a_params* p) {
   p->ext_levels = -1;
   p->translation = 1;
   p->strict = 0;
+  p->tail_waves = 0;
+  p->tail_budget = 0;
 }
 
 namespace {
@@ -720,6 +775,10 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     delete b;
     return fail(C2A_ERR_BAD_ARG, "strict must be 0 or 1");
   }
+  if (params && params->tail_waves != 0 && params->tail_waves != kTailWaves) {
+    delete b;
+    return fail(C2A_ERR_BAD_ARG, "tail_waves must be 0 or 4");
+  }
   try {
     if ((rc = make_mesh_table(meshes, nmesh, b->mtab)) ||
         (rc = prepare_queries(b->mtab, q, nq, params && params->strict, hq, pairs))) {
@@ -802,6 +861,14 @@ int c2a_batch_create(c2a_mesh* const* meshes, int32_t nmesh, const c2a_query* q,
     return hip_fail(e, "batch upload");
   }
   fill_args(b, params);
+  {
+    c2a_params def;
+    c2a_params_default(&def);
+    if ((rc = setup_tail(b, params ? params : &def))) {
+      delete b;
+      return rc;
+    }
+  }
   *out = b;
   return C2A_OK;
 }
@@ -816,7 +883,7 @@ int c2a_batch_run(c2a_batch* b, void* stream) {
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(b->ev0, s));
-  int rc = run_kernel(b->args, b->K, b->CAP, b->NW, b->blocks, s);
+  int rc = run_batch(b, s);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(b->ev1, s));
   b->timed = true;
@@ -838,7 +905,7 @@ int c2a_batch_run_graph(c2a_batch* b, void* stream) {
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     int rc = C2A_OK;
     if (e == hipSuccess) {
-      rc = run_kernel(b->args, b->K, b->CAP, b->NW, b->blocks, cs);
+      rc = run_batch(b, cs);
       hipGraph_t g = nullptr;
       const hipError_t e2 = hipStreamEndCapture(cs, &g);
       if (!rc && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
@@ -860,6 +927,16 @@ int c2a_batch_run_graph(c2a_batch* b, void* stream) {
   HIP_TRY(hipGraphLaunch(b->graph, s));
   HIP_TRY(hipEventRecord(b->ev1, s));
   b->timed = true;
+  return C2A_OK;
+}
+
+int c2a_batch_tail_count(c2a_batch* b, int64_t* n) {
+  if (!b || !n) return fail(C2A_ERR_BAD_ARG, "null argument");
+  *n = 0;
+  if (!b->tail) return C2A_OK;
+  int32_t v = 0;
+  HIP_TRY(hipMemcpy(&v, b->args.tail_count, sizeof(v), hipMemcpyDeviceToHost));
+  *n = v;
   return C2A_OK;
 }
 

@@ -1859,6 +1859,33 @@ C2A_HD void finish_query(Wave<K, CAP>& g, Leader& L, const KArgs& a, bool collis
   L.state = ST_FETCH;
 }
 
+// Saves the CA-loop state of a query about to start its next step and hands
+// the query to the tail pass (the next step starts there without a replay:
+// pair values do not depend on it).
+C2A_HD void hand_over(Leader& L, const KArgs& a, int lane) {
+  if (lane == 0) {
+    TailRec& t = a.tail_rec[L.qi];
+    t.lamda = L.lamda;
+    t.lastLamda = L.lastLamda;
+    t.Up = L.Up;
+    t.M = L.M;
+    for (int i = 0; i < 3; i++) { t.p1[i] = L.p1[i]; t.p2[i] = L.p2[i]; }
+    t.bv = L.bv;
+    t.tri = L.tri;
+    t.nItrs = L.nItrs;
+    t.numCA = L.numCA;
+    t.flags = L.flags;
+    t.last1 = L.last1;
+    t.last2 = L.last2;
+#if defined(__HIP_DEVICE_COMPILE__)
+    a.tail_list[atomicAdd(a.tail_count, 1)] = (int32_t)L.qi;
+#else
+    a.tail_list[(*a.tail_count)++] = (int32_t)L.qi;
+#endif
+  }
+  L.state = ST_FETCH;
+}
+
 // End of a step in the CA loop (C2A.cpp:2029-2123): schedules the next step
 // (state ST_STEP) or finishes the query.
 template <int K, int CAP>
@@ -1914,6 +1941,7 @@ C2A_HD void after_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
         L.numCA++;
         L.Up = 1.0 - L.lamda;
         L.state = ST_STEP;
+        if (a.tail_budget > 0 && !a.tail_pass && L.bv >= a.tail_budget) hand_over(L, a, lane);
         return;
       }
     }
@@ -1941,7 +1969,13 @@ C2A_HD int64_t take_ticket(Wave<K, CAP>& g, int64_t* counter, int lane) {
 template <int K, int CAP>
 C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   int64_t qi = take_ticket(g, a.counter, lane);
-  if (qi >= a.nq) {
+  if (a.tail_pass) {   // the first launch's hand-overs
+    if (qi >= *a.tail_count) {
+      L.state = ST_DONE;
+      return false;
+    }
+    qi = a.tail_list[qi];
+  } else if (qi >= a.nq) {
     L.state = ST_DONE;
     return false;
   }
@@ -2112,6 +2146,22 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   if (unid(g.mo[0].angVel) < 1e-8 && unid(g.mo[1].angVel) < 1e-8)
     L.flags |= C2A_FLAG_TRANSLATION_ONLY;
   L.state = (a.translation && (L.flags & C2A_FLAG_TRANSLATION_ONLY)) ? ST_TRANS : ST_STEP;
+  if (a.tail_pass) {   // continue where the first launch stopped
+    const TailRec& t = a.tail_rec[qi];
+    L.lamda = t.lamda;
+    L.lastLamda = t.lastLamda;
+    L.Up = t.Up;
+    L.M = t.M;
+    for (int i = 0; i < 3; i++) { L.p1[i] = t.p1[i]; L.p2[i] = t.p2[i]; }
+    L.bv = t.bv;
+    L.tri = t.tri;
+    L.nItrs = t.nItrs;
+    L.numCA = t.numCA;
+    L.flags = t.flags;
+    L.last1 = t.last1;
+    L.last2 = t.last2;
+    L.state = ST_STEP;
+  }
   return true;
 }
 

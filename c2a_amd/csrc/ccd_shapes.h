@@ -12,12 +12,13 @@
 
 // X(K, CAP, NW, LRN): K = speculative subtree depth, CAP = DFS stack entries,
 // NW = wavefronts per query, LRN = learned child order.  CAP 192 (hierarchy
-// pairs up to 190 levels deep, 52 KB of LDS) is instantiated at K = 5 only.
+// pairs up to 190 levels deep, 52 KB of LDS) is instantiated at K = 5 only;
+// NW = 4 (the tail pass) at K = 5, plain order, CAP 48 and 96.
 #define C2A_SHAPES(X)                                                                  \
   X(1, 48, 1, 0) X(2, 48, 1, 0) X(3, 48, 1, 0) X(4, 48, 1, 0) X(5, 48, 1, 0)            \
   X(6, 48, 1, 0) X(1, 96, 1, 0) X(2, 96, 1, 0) X(3, 96, 1, 0) X(4, 96, 1, 0)            \
   X(5, 96, 1, 0) X(6, 96, 1, 0) X(5, 48, 1, 1) X(5, 96, 1, 1) X(5, 192, 1, 0)          \
-  X(5, 192, 1, 1)
+  X(5, 192, 1, 1) X(5, 48, 4, 0) X(5, 96, 4, 0)
 
 namespace c2a {
 

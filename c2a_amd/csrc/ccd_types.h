@@ -30,6 +30,16 @@ struct QueryDev {
   int32_t seed[2];
 };
 
+// The CA-loop state of a query between two steps (C2A.cpp:2075-2123): what
+// the tail pass needs to continue it.  Everything else of the next step is
+// recomputed from the query and these values.
+struct TailRec {
+  double lamda, lastLamda, Up, M;   // M: the last step's mint (error schedule)
+  double p1[3], p2[3];              // closest points of the last improving leaf
+  int64_t bv, tri;
+  int32_t nItrs, numCA, flags, last1, last2, pad[3];
+};
+
 // One explicit CA step (backs C2A_TimeOfContactStep).
 struct StepDev {
   Motion mo[2];
@@ -215,6 +225,17 @@ struct KArgs {
   int64_t* sample_keys;    // sample_cap: query << 34 | CA step << 26 | record
   int64_t sample_cap;
   unsigned long long* sample_count;   // rows claimed (may exceed sample_cap)
+  // Tail pass: the first launch hands a query whose BV tests have reached
+  // tail_budget at the end of a CA step over to a second launch, which
+  // continues it from its saved CA-loop state with several wavefronts per
+  // query.  The batch time is its slowest query's; the second launch gives
+  // the few long queries the lanes the others no longer use.
+  int64_t tail_budget;     // 0: one launch
+  int32_t tail_pass;       // 1 in the second launch: queries come from tail_list
+  int32_t pad2;
+  TailRec* tail_rec;       // per query
+  int32_t* tail_list;      // queries handed over, in hand-over order
+  int32_t* tail_count;     // their number
 };
 
 // splitmix64 finaliser: the sample selection hash.

@@ -108,6 +108,17 @@ typedef struct {
                                translation-only queries take the rotation
                                path (the translation branch ignores object
                                2's velocity, C2A.cpp:1472-1474) */
+  int32_t tail_waves;       /* wavefronts per query in the tail pass; this
+                               build provides 4 (0 = 4) */
+  int64_t tail_budget;      /* tail pass: a query that has made this many BV
+                               tests when a CA step ends continues, from
+                               that step on, in a second launch with
+                               tail_waves wavefronts per query.  Results are
+                               unchanged.  0 (default) = one launch.  Plain
+                               order without sample logging only (otherwise
+                               one launch).  It pays only when a few queries
+                               run far longer than all others: the second
+                               launch starts when the first one ends. */
 } c2a_params;
 
 enum { C2A_ORDER_PLAIN = 0, C2A_ORDER_LEARNED = 1 };
@@ -244,6 +255,10 @@ int c2a_batch_destroy(c2a_batch *b);
 /* Kernel-only device time of the last c2a_batch_run, in milliseconds,
  * measured with HIP events on the stream the kernel was launched on. */
 int c2a_batch_last_ms(c2a_batch *b, float *ms);
+/* Queries the last run handed to the tail pass (c2a_params.tail_budget);
+ * waits for the batch's stream.  0 when the batch runs in one launch.  No
+ * counterpart in the reference (diagnostics of this build's launches). */
+int c2a_batch_tail_count(c2a_batch *b, int64_t *n);
 /* Training samples logged by the last c2a_batch_run (params->sample_cap > 0;
  * replay_log > 0): every CA step after the first replays the previous
  * step's traversal, and each replayed pair whose replayed subtree holds a

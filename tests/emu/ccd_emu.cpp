@@ -36,6 +36,12 @@ static int64_t g_rounds = 0;   // speculative rounds of the last batch
 extern "C" int64_t emu_rounds() { return g_rounds; }
 static int emu_arena_rounds = 1;   // the CPU harness exercises the arena-round path
 extern "C" void emu_set_arena_rounds(int on) { emu_arena_rounds = on; }
+// Tail pass: queries reaching this many BV tests at the end of a CA step are
+// handed over and continued by a second run (0 = one run).
+static int64_t emu_tail_budget = 0;
+static int64_t emu_tail_handed = 0;
+extern "C" void emu_set_tail_budget(int64_t b) { emu_tail_budget = b; }
+extern "C" int64_t emu_tail_count() { return emu_tail_handed; }
 extern "C" int64_t emu_round_kind(int later, int kind) { return g_kind[later][kind]; }
 
 // Learned order on the CPU: a host copy of a scorer (scalar forward pass, so
@@ -220,15 +226,34 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
   // stack capacity as the library picks it (the deepest one at depth 5)
   const int need = b1.max_depth + b2.max_depth + 2;
   const bool small = need <= 48;
-  if (need > 96) return need <= 192 ? run<5, 192>(a) : 2;
-  switch (K) {
-    case 1: return small ? run<1, 48>(a) : run<1, 96>(a);
-    case 2: return small ? run<2, 48>(a) : run<2, 96>(a);
-    case 3: return small ? run<3, 48>(a) : run<3, 96>(a);
-    case 4: return small ? run<4, 48>(a) : run<4, 96>(a);
-    case 5: return small ? run<5, 48>(a) : run<5, 96>(a);
-    default: return small ? run<6, 48>(a) : run<6, 96>(a);
-  }
+  if (need > 96) K = 0;
+  auto run_shape = [&](const KArgs& ka) {
+    if (K == 0) return need <= 192 ? run<5, 192>(ka) : 2;
+    switch (K) {
+      case 1: return small ? run<1, 48>(ka) : run<1, 96>(ka);
+      case 2: return small ? run<2, 48>(ka) : run<2, 96>(ka);
+      case 3: return small ? run<3, 48>(ka) : run<3, 96>(ka);
+      case 4: return small ? run<4, 48>(ka) : run<4, 96>(ka);
+      case 5: return small ? run<5, 48>(ka) : run<5, 96>(ka);
+      default: return small ? run<6, 48>(ka) : run<6, 96>(ka);
+    }
+  };
+  emu_tail_handed = 0;
+  if (emu_tail_budget <= 0) return run_shape(a);
+  // two runs, as the library's two launches
+  std::vector<TailRec> rec((size_t)nq);
+  std::vector<int32_t> list((size_t)nq + 1);
+  int32_t count = 0;
+  a.tail_budget = emu_tail_budget;
+  a.tail_rec = rec.data();
+  a.tail_list = list.data();
+  a.tail_count = &count;
+  int rc = run_shape(a);
+  if (rc) return rc;
+  emu_tail_handed = count;
+  counter = 0;
+  a.tail_pass = 1;
+  return run_shape(a);
 }
 
 // Static distance queries (MODE_DISTANCE) between two meshes, one lane.

@@ -214,3 +214,37 @@ def test_kernel_logic_deep_hierarchies(emu):
     ref = O.batch_toc(m1, m2, *P, 1e-6, 1e-6, seeds=seeds, with_contacts=True)
     got = run_emu(emu, t1, t2, P, G.default_params(tol_d=1e-6, tol_t=1e-6), 5, seeds=seeds)
     same(ref, got)
+
+
+@pytest.mark.parametrize("lanes", [1, 64])
+@pytest.mark.parametrize("budget", [50, 3000])
+def test_kernel_logic_tail_hand_over(emu, lanes, budget):
+    """The tail pass's hand-over: a first run hands every query that has made
+    `budget` BV tests at the end of a CA step over (saving its CA-loop
+    state), a second run continues them without a replay for their first
+    step.  Results equal the oracle's, with one lane and with 64 (the
+    second run here also has one wavefront; the kernel's tail pass has
+    four, whose code is the same)."""
+    lib = emu if lanes == 1 else C.CDLL(os.path.join(os.path.dirname(EMU), "libccd_emu64.so"))
+    lib.emu_set_tail_budget.argtypes = [C.c_int64]
+    lib.emu_tail_count.restype = C.c_int64
+    tris, _ = W.torus_knot(n_seg=64, n_ring=8)
+    t9 = tris.reshape(-1, 9)
+    nq = 16
+    P = W.knot_queries(nq, 123, 0.4)
+    ref = O.batch_toc(O.OModel(t9), O.OModel(t9), *P, 1e-3, 1e-4,
+                      seeds=np.zeros((nq, 2), np.int32), with_contacts=True)
+    try:
+        lib.emu_set_tail_budget(budget)
+        for strict in (0, 1):
+            r = ref
+            if strict:
+                with O.strict_mode():
+                    r = O.batch_toc(O.OModel(t9), O.OModel(t9), *P, 1e-3, 1e-4,
+                                    seeds=np.zeros((nq, 2), np.int32), with_contacts=True)
+            got = run_emu(lib, t9, t9, P, G.default_params(tol_d=1e-3, tol_t=1e-4, strict=strict),
+                          5)
+            same(r, got)
+            assert 0 < lib.emu_tail_count() <= int((r["num_bv_tests"] >= budget).sum())
+    finally:
+        lib.emu_set_tail_budget(0)
