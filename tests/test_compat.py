@@ -82,7 +82,7 @@ def test_solve_demo_matches_oracle(tmp_path):
     tris, _ = W.torus_knot()
     t9 = np.ascontiguousarray(tris.reshape(-1, 9))
     cfg = W.CONFIG_KNOT_SOLVE
-    nq = 300
+    nq = cfg["nq"]   # all 1000 queries
     P = [p[:nq] for p in W.knot_queries(cfg["nq"], cfg["seed"], collide_frac=0.0)]
     poses = np.stack(P, axis=1)            # nq x 4 x 12
     payload = (_i32(len(t9)) + t9.tobytes() + _i32(len(t9)) + t9.tobytes() + _i32(nq) +

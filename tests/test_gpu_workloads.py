@@ -24,10 +24,11 @@ def check(ref, got, label):
     assert np.array_equal(ref["out_last_triB"], got["last_tri"][:, 1]), label
 
 
-def test_uneven_mesh_table_bit_exact():
+@pytest.mark.parametrize("nq", [384, W.CONFIG_UNEVEN["nq"]])
+def test_uneven_mesh_table_bit_exact(nq):
+    """The first 384 queries, then the whole 16,384-query batch."""
     tab = W.uneven_meshes()
     sizes = [len(t) for t, _, _ in tab]
-    nq = 384
     P00, P01, P10, P11, mp = W.uneven_queries(nq, W.CONFIG_UNEVEN["seed"], sizes,
                                               [r for _, _, r in tab])
     meshes = [G.Mesh(t, vidx=idx) for t, idx, _ in tab]
@@ -39,7 +40,7 @@ def test_uneven_mesh_table_bit_exact():
     for a, b in sorted(set(map(tuple, mp))):
         sel = np.nonzero((mp[:, 0] == a) & (mp[:, 1] == b))[0]
         r = O.batch_toc(models[a], models[b], P00[sel], P01[sel], P10[sel], P11[sel], 1e-3, 1e-4,
-                        seeds=np.zeros((len(sel), 2), np.int32), with_contacts=True, nthreads=8)
+                        seeds=np.zeros((len(sel), 2), np.int32), with_contacts=True, nthreads=16)
         if refs is None:
             refs = np.zeros(nq, dtype=r.dtype)
         refs[sel] = r
@@ -48,11 +49,12 @@ def test_uneven_mesh_table_bit_exact():
     assert (got["collisionfree"] == 0).any() and (got["collisionfree"] == 1).any()
 
 
-def test_grazing_262k_bit_exact():
+@pytest.mark.parametrize("nq", [64, W.CONFIG_GRAZING["nq"]])
+def test_grazing_262k_bit_exact(nq):
+    """The first 64 passes, then the whole 1024-query batch."""
     cfg = W.CONFIG_GRAZING
     s, idx = W.uv_sphere(cfg["n_lon"], cfg["n_lat"], cfg["noise"], 3)
     t9 = s.reshape(-1, 9)
-    nq = 64
     P = W.grazing_queries(nq, cfg["seed"])
     meshes = [G.Mesh(t9, vidx=idx), G.Mesh(t9, vidx=idx)]
     print("grazing: %d queries, %d triangles per sphere" % (nq, len(t9)), flush=True)
@@ -60,7 +62,7 @@ def test_grazing_262k_bit_exact():
                                                                     tol_t=cfg["tol_t"]))
     m1, m2 = O.OModel(t9), O.OModel(t9)
     ref = O.batch_toc(m1, m2, *P, cfg["tol_d"], cfg["tol_t"], seeds=np.zeros((nq, 2), np.int32),
-                      with_contacts=True, nthreads=8)
+                      with_contacts=True, nthreads=16)
     check(ref, got, "grazing")
 
 
