@@ -313,6 +313,8 @@ struct Leader {
   int kdepth, kmiss, klog;       // this query's speculative depth, depth below
                                  // a miss and replay log capacity
   int kext;                      // levels of speculative extension in the replay
+  int kx;                        // ... this step: kext, or 0 after a step with few misses
+  int nmiss;                     // misses the scan met in this step
   unsigned char* ab;             // this wavefront's replay arena
 };
 
@@ -1146,10 +1148,10 @@ C2A_HD void replay(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl)
   // replayed instead of walking below them one round at a time.  Values never
   // depend on the bound, so this changes no result.
   int ntot = n;
-  if (L.kext > 0) {
+  if (L.kx > 0) {
     LogEnt C2A_GLOBAL* lgw = const_cast<LogEnt C2A_GLOBAL*>(lg);
     int lo = 1, hi = nrec;
-    for (int pass = 0; pass < L.kext; pass++) {
+    for (int pass = 0; pass < L.kx; pass++) {
       if (lane == 0) g.next = 0;
       lane_sync();
       const int room = L.klog - ntot;
@@ -1435,6 +1437,7 @@ C2A_HD int scan(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane, int nl) {
         continue;
       }
       // a pair the previous step did not expand: sequential walk below it.
+      L.nmiss++;
       // If the last round was the one below an earlier miss of this window
       // and covered this one too, its heap is still in the slots; otherwise
       // this miss and the misses after it in the window are gathered for
@@ -1784,6 +1787,13 @@ C2A_HD void begin_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   L.nflush = 0;
   L.pq_pending = 0;
   L.napp = a.logcap > 0 ? L.nlog_prev : 0;   // round records follow the replayed ones
+  // The replay's extension removes misses at the price of records the step
+  // may not expand.  Adaptive (the autotuned levels): it stays off for a
+  // step after one whose misses were fewer than one in 64 logged expansions
+  // (a step without a scan, the first one, counts as many).  Values never
+  // depend on it.
+  L.kx = (!a.ext_adaptive || L.nmiss < 0 || 64 * L.nmiss >= L.nlog_prev) ? L.kext : 0;
+  L.nmiss = (a.logcap > 0 && L.nlog_prev > 0) ? 0 : -1;
   L.wbase = -(1 << 30);
   // initial bound from the caller-seeded pair (C2A.cpp:1812-1858)
   double p[3], q[3], ta[9], tb[9];
@@ -1984,6 +1994,7 @@ C2A_HD bool fetch(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   L.kmiss = a.miss_depth;
   L.klog = a.logcap;
   L.kext = a.ext_levels;
+  L.nmiss = -1;
   L.flags = 0;
   L.bv = 0;
   L.tri = 0;

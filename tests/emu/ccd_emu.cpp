@@ -41,6 +41,10 @@ extern "C" void emu_set_arena_rounds(int on) { emu_arena_rounds = on; }
 static int64_t emu_tail_budget = 0;
 static int64_t emu_tail_handed = 0;
 extern "C" void emu_set_tail_budget(int64_t b) { emu_tail_budget = b; }
+// The extension switched per step by the previous step's misses (the
+// library's mode for the autotuned levels).
+static int emu_ext_adaptive = 0;
+extern "C" void emu_set_ext_adaptive(int on) { emu_ext_adaptive = on; }
 extern "C" int64_t emu_tail_count() { return emu_tail_handed; }
 extern "C" int64_t emu_round_kind(int later, int kind) { return g_kind[later][kind]; }
 
@@ -212,6 +216,7 @@ extern "C" int emu_ccd_batch(const double* tris1, int64_t n1, const double* tris
   a.arena_rounds = emu_arena_rounds;
   a.miss_depth = p->miss_depth;
   a.ext_levels = p->ext_levels < 0 ? 0 : p->ext_levels;
+  a.ext_adaptive = emu_ext_adaptive;
   a.strict = p->strict != 0;
   a.translation = p->translation && !a.strict;
   a.scorer = g_scorer;

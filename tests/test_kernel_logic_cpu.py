@@ -59,14 +59,22 @@ def test_kernel_logic_knot(emu, K):
     same(ref, got)
 
 
-@pytest.mark.parametrize("ext", [1, 3])
-def test_kernel_logic_replay_extension(emu, ext):
-    """Pairs computed ahead below replayed pairs (ext_levels) change no result."""
+@pytest.mark.parametrize("ext,adaptive", [(1, 0), (3, 0), (1, 1), (2, 1)])
+def test_kernel_logic_replay_extension(emu, ext, adaptive):
+    """Pairs computed ahead below replayed pairs (ext_levels) change no result,
+    at every step or switched per step by the previous step's misses (the
+    library's mode for the autotuned levels)."""
     tris, _ = W.torus_knot(n_seg=128, n_ring=12)
     t9 = tris.reshape(-1, 9)
     nq = 24
     P = W.knot_queries(nq, 610 + ext, collide_frac=0.4)
-    got = run_emu(emu, t9, t9, P, G.default_params(tol_d=1e-3, tol_t=1e-4, ext_levels=ext), 5)
+    emu.emu_set_ext_adaptive.argtypes = [C.c_int]
+    emu.emu_set_ext_adaptive(adaptive)
+    try:
+        got = run_emu(emu, t9, t9, P, G.default_params(tol_d=1e-3, tol_t=1e-4, ext_levels=ext),
+                      5)
+    finally:
+        emu.emu_set_ext_adaptive(0)
     m1, m2 = O.OModel(t9), O.OModel(t9)
     ref = O.batch_toc(m1, m2, *P, 1e-3, 1e-4, seeds=np.zeros((nq, 2), np.int32),
                       with_contacts=True, nthreads=4)

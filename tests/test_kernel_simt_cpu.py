@@ -44,6 +44,7 @@ def emu64():
     L = C.CDLL(os.environ.get("CCD_EMU64", os.path.join(HERE, "emu", "libccd_emu64.so")))
     L.emu_set_arena_rounds.argtypes = [C.c_int]
     L.emu_set_scorer.argtypes = [C.c_void_p]
+    L.emu_set_ext_adaptive.argtypes = [C.c_int]
     yield L
     L.emu_set_arena_rounds(1)
     L.emu_set_scorer(None)
@@ -72,6 +73,7 @@ CASES = [
     dict(K=5, arena=1, replay_log=0),       # no replay at all
     dict(K=5, arena=1, ext_levels=1),
     dict(K=4, arena=0, ext_levels=2),
+    dict(K=5, arena=0, ext_levels=1, adaptive=1),   # extension switched per step
     dict(K=5, arena=0, miss_depth=1),       # 16 misses per shared round
     dict(K=5, arena=0, miss_depth=3),       # 4 per round
     dict(K=4, arena=0, miss_depth=3),       # 2 per round
@@ -82,10 +84,14 @@ CASES = [
 def test_simt_bit_exact(emu64, knot, case):
     t9, P, ref = knot
     c = dict(case)
-    K, arena = c.pop("K"), c.pop("arena")
+    K, arena, adaptive = c.pop("K"), c.pop("arena"), c.pop("adaptive", 0)
     emu64.emu_set_arena_rounds(arena)
-    got = run_emu(emu64, t9, t9, P, G.default_params(tol_d=1e-3, tol_t=1e-4, spec_depth=K, **c),
-                  K)
+    emu64.emu_set_ext_adaptive(adaptive)
+    try:
+        got = run_emu(emu64, t9, t9, P,
+                      G.default_params(tol_d=1e-3, tol_t=1e-4, spec_depth=K, **c), K)
+    finally:
+        emu64.emu_set_ext_adaptive(0)
     same(ref, got)
 
 
