@@ -239,7 +239,7 @@ void params_to_args(KArgs& a, const c2a_params* p) {
   a.mode = MODE_QUERY;
   a.miss_depth = p->miss_depth < 0 ? 0 : p->miss_depth;
   a.ext_levels = p->ext_levels < 0 ? 0 : (p->ext_levels > 8 ? 8 : p->ext_levels);
-  a.ext_adaptive = p->ext_levels < 0 ? 1 : 0;   // the buckets' levels, switched per step
+  a.ext_adaptive = 1;   // switched per step by the previous step's misses
   a.strict = p->strict != 0 ? 1 : 0;
   a.translation = (p->translation != 0 && !a.strict) ? 1 : 0;
 }

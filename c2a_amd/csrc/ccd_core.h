@@ -1788,7 +1788,7 @@ C2A_HD void begin_step(Wave<K, CAP>& g, Leader& L, const KArgs& a, int lane) {
   L.pq_pending = 0;
   L.napp = a.logcap > 0 ? L.nlog_prev : 0;   // round records follow the replayed ones
   // The replay's extension removes misses at the price of records the step
-  // may not expand.  Adaptive (the autotuned levels): it stays off for a
+  // may not expand.  Adaptive (as the library runs it): it stays off for a
   // step after one whose misses were fewer than one in 64 logged expansions
   // (a step without a scan, the first one, counts as many).  Values never
   // depend on it.

@@ -202,7 +202,8 @@ struct KArgs {
   int32_t miss_depth;      // depth of the first round below a miss (0 = K)
   int32_t ext_levels;      // replay: levels of speculative extension below
                            // pairs that newly pass the descend test
-  int32_t ext_adaptive;    // 1: a step after one with few misses skips it
+  int32_t ext_adaptive;    // 1 (the library): a step after one with few misses
+                           // skips it; 0 (test harness): every step
   int32_t translation;     // C2A_Solve's translation-only branch for queries
                            // whose two rotations are below 1e-8
   int32_t strict;          // strict (conservative) mode: every tested leaf

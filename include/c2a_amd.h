@@ -88,10 +88,10 @@ typedef struct {
                                at the step's initial bound; -1 (default) = the
                                autotuned value of each query's size bucket
                                when spec_depth is 0 (0 under the learned
-                               order or with sample logging), else 0, and
-                               a query's step skips it after a step with
-                               fewer misses than one in 64 expansions;
-                               0 = off; N > 0: N levels at every step */
+                               order or with sample logging), else 0;
+                               0 = off.  A query's step skips it after a
+                               step with fewer misses than one in 64
+                               expansions. */
   int32_t translation;      /* 1 (default): queries whose two rotations are
                                below 1e-8 take C2A_Solve's translation-only
                                branch (C2A.cpp:1362-1521, 2032-2049: one
