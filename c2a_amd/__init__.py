@@ -106,7 +106,7 @@ EXPORTS = ["c2a_params_default", "c2a_mesh_create", "c2a_mesh_destroy", "c2a_mes
            "c2a_batch_samples", "c2a_ccd_batch_noalloc", "c2a_contact_list",
            "c2a_batch_run_graph", "c2a_distance_queue", "c2a_distance_contact_space",
            "c2a_tolerance", "c2a_collide_pairs", "c2a_mesh_create_device", "c2a_mesh_save",
-           "c2a_mesh_load", "c2a_batch_tail_count"]
+           "c2a_mesh_load", "c2a_batch_tail_count", "c2a_mesh_refit"]
 
 
 def lib():
@@ -126,6 +126,7 @@ def lib():
     L.c2a_mesh_destroy.argtypes = [vp]
     L.c2a_mesh_create_device.argtypes = [dp, ip, ip, i64, vp, C.POINTER(vp)]
     L.c2a_mesh_save.argtypes = [vp, C.c_char_p]
+    L.c2a_mesh_refit.argtypes = [vp, dp, C.c_int32, vp]
     L.c2a_mesh_load.argtypes = [C.c_char_p, dp, ip, ip, i64, C.POINTER(vp)]
     L.c2a_mesh_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), ip]
     L.c2a_mesh_tri_ids.argtypes = [vp, ip]
@@ -248,6 +249,14 @@ class Mesh:
         tris = np.zeros((self.ntri, 9))
         _check(lib().c2a_mesh_export(self.handle, _dp(nodes), _dp(tris)))
         return nodes, tris
+
+    def refit(self, tris, device=True):
+        """Refits every node's volume to moved triangles (c2a_mesh_refit): the
+        same triangles, in the order given at creation, n x 9."""
+        tris = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+        if tris.shape[0] != self.ntri:
+            raise ValueError("refit takes the mesh's %d triangles" % self.ntri)
+        _check(lib().c2a_mesh_refit(self.handle, _dp(tris), 1 if device else 0, None))
 
     def save(self, path):
         """Writes the hierarchy file (c2a_mesh_save)."""

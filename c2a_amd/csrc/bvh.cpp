@@ -23,7 +23,10 @@ struct Tri {
   double p[9];
   int32_t id;
   int32_t vidx[3];
+  int32_t src;   // caller's index
 };
+
+void make_parent_relative_nodes(std::vector<GpuNode>& nodes);
 
 // vertex covariance (C2A_Build.cpp:279-335); acc[0..2] is the vertex sum
 void covariance(double* M, double* acc, const Tri* t, int n) {
@@ -148,9 +151,13 @@ struct Builder {
     }
   }
 
-  // make_parent_relative (C2A_Build.cpp:493-544): post-order, each node made
-  // relative to its parent's model-frame R/Tr.
-  void make_parent_relative() {
+  void make_parent_relative() { make_parent_relative_nodes(nodes); }
+};
+
+// make_parent_relative (C2A_Build.cpp:493-544): post-order, each node made
+// relative to its parent's model-frame R/Tr.
+void make_parent_relative_nodes(std::vector<GpuNode>& nodes) {
+  {
     struct Item { int bn; bool expanded; double pR[9], pT[3]; };
     std::vector<Item> st;
     Item root{0, false, {1, 0, 0, 0, 1, 0, 0, 0, 1}, {0, 0, 0}};
@@ -171,7 +178,7 @@ struct Builder {
       st.pop_back();
     }
   }
-};
+}
 
 }  // namespace
 
@@ -185,6 +192,7 @@ int build_bvh(const double* xyz9, const int32_t* ids, const int32_t* vidx3, int6
     Tri& t = B.tris[(size_t)i];
     for (int k = 0; k < 9; k++) t.p[k] = xyz9[9 * i + k];
     t.id = ids ? ids[i] : (int32_t)i;
+    t.src = (int32_t)i;
     for (int k = 0; k < 3; k++) t.vidx[k] = vidx3 ? vidx3[3 * i + k] : (int32_t)(3 * i + k);
   }
   B.nodes.assign((size_t)(2 * n - 1), GpuNode{});
@@ -196,13 +204,67 @@ int build_bvh(const double* xyz9, const int32_t* ids, const int32_t* vidx3, int6
   out->tris.resize((size_t)n * 9);
   out->tri_id.resize((size_t)n);
   out->tri_vidx.resize((size_t)n * 3);
+  out->src_index.resize((size_t)n);
   for (int64_t i = 0; i < n; i++) {
     const Tri& t = B.tris[(size_t)i];
     for (int k = 0; k < 9; k++) out->tris[(size_t)i * 9 + k] = t.p[k];
     out->tri_id[(size_t)i] = t.id;
     for (int k = 0; k < 3; k++) out->tri_vidx[(size_t)i * 3 + k] = t.vidx[k];
+    out->src_index[(size_t)i] = t.src;
   }
   out->max_depth = B.max_depth;
+  return 0;
+}
+
+void node_ranges(const std::vector<GpuNode>& nodes, std::vector<int32_t>& first,
+                 std::vector<int32_t>& num, std::vector<int32_t>& depth) {
+  const size_t nn = nodes.size();
+  first.assign(nn, 0);
+  num.assign(nn, 0);
+  depth.assign(nn, 0);
+  if (nn == 0) return;
+  // pre-order for the depths, then children after parents reversed for the
+  // ranges (a child's number is always above its parent's)
+  for (size_t v = 0; v < nn; v++) {
+    const int fc = nodes[v].first_child;
+    if (fc >= 0) depth[(size_t)fc] = depth[(size_t)fc + 1] = depth[v] + 1;
+  }
+  for (size_t v = nn; v-- > 0;) {
+    const int fc = nodes[v].first_child;
+    if (fc < 0) {
+      first[v] = -fc - 1;
+      num[v] = 1;
+    } else {
+      first[v] = first[(size_t)fc];
+      num[v] = num[(size_t)fc] + num[(size_t)fc + 1];
+    }
+  }
+}
+
+int refit_bvh(Bvh* b, const double* xyz9) {
+  const size_t n = b->tri_id.size();
+  if (n == 0 || b->src_index.size() != n) return -1;
+  std::vector<Tri> tris(n);
+  for (size_t i = 0; i < n; i++) {
+    const int64_t s = b->src_index[i];
+    for (int k = 0; k < 9; k++) tris[i].p[k] = xyz9[9 * s + k];
+  }
+  std::vector<int32_t> first, num, depth;
+  node_ranges(b->nodes, first, num, depth);
+  std::vector<double> scratch;
+  for (size_t v = 0; v < b->nodes.size(); v++) {
+    GpuNode& nd = b->nodes[v];
+    double C[9], acc[9], E[9], sv[3], R[9];
+    covariance(C, acc, &tris[(size_t)first[v]], num[v]);
+    jacobi3(C, E, sv);
+    orient_from_eigen(E, sv, R);
+    fit_rss(nd, R, &tris[(size_t)first[v]], num[v], scratch);
+    for (int k = 0; k < 9; k++) nd.Rloc[k] = nd.R[k];
+  }
+  make_parent_relative_nodes(b->nodes);
+  for (auto& nd : b->nodes) nd.size = node_size(nd.l, nd.r);
+  for (size_t i = 0; i < n; i++)
+    for (int k = 0; k < 9; k++) b->tris[i * 9 + k] = tris[i].p[k];
   return 0;
 }
 

@@ -32,11 +32,23 @@ struct Bvh {
   std::vector<double> tris;       // n * 9, reordered by the build
   std::vector<int32_t> tri_id;    // caller id of each reordered triangle
   std::vector<int32_t> tri_vidx;  // n * 3 vertex indices (reordered)
+  std::vector<int32_t> src_index; // caller's index of each reordered triangle
   int max_depth = 0;
 };
 
 // Builds the hierarchy.  Returns 0, or a negative code for an empty model.
 int build_bvh(const double* xyz9, const int32_t* ids, const int32_t* vidx3, int64_t n, Bvh* out);
+
+// Refit: the tree, the triangle order and the node numbers stay; every
+// node's volume is fitted again, with the build's fit, to the triangles
+// xyz9 (n x 9, the caller's order of the build).  Each node is fitted over
+// its triangles in their final order.  Returns 0, or -1 without a
+// src_index.
+int refit_bvh(Bvh* b, const double* xyz9);
+
+// First triangle position and triangle count of every node's subtree.
+void node_ranges(const std::vector<GpuNode>& nodes, std::vector<int32_t>& first,
+                 std::vector<int32_t>& num, std::vector<int32_t>& depth);
 
 // Jacobi eigen-decomposition used by the builder (exported for tests).
 void jacobi_eigen3(const double* A, double* V, double* d);

@@ -16,4 +16,10 @@ hipError_t build_bvh_device(const double* xyz9, const int32_t* ids, const int32_
                             int64_t n, Bvh* out, void** nodes_dev, void** tris_dev,
                             hipStream_t s);
 
+// Refits the hierarchy *b (host copy) and its device copies to the
+// triangles xyz9 (host, the caller's order of the build): refit_bvh's
+// result, computed on the device, copied back into *b.
+hipError_t refit_bvh_device(const double* xyz9, Bvh* b, void* nodes_dev, void* tris_dev,
+                            hipStream_t s);
+
 }  // namespace c2a

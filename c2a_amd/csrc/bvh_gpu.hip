@@ -17,6 +17,9 @@
 // Node numbers follow the host's depth-first order through dfs_first_child.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
+
+#include <vector>
 
 #include "bvh.h"
 #include "bvh_fit.h"
@@ -48,6 +51,7 @@ struct BuildArgs {
   const BuildJob* jobs[3];  // this level's nodes of each class
   BuildJob* next[3];        // the next level's nodes of each class
   int32_t* counts;          // [0..2] next level's node counts, [3] max depth
+  int32_t refit;            // 1: fit the listed nodes only (topology kept)
 };
 
 __device__ inline void push_child(const BuildArgs& a, const BuildJob& j) {
@@ -307,15 +311,18 @@ __global__ void __launch_bounds__(B) build_node(BuildArgs a, int cls) {
     nd.angR = angR;
     fit_finish(Rr, minx, maxx, miny, maxy, cz, r, nd.Tr, nd.l, &nd.r);
     for (int k = 0; k < 9; k++) nd.Rloc[k] = Rr[k];
-    nd.depth = job.depth;
-    nd.first_child = dfs_first_child(job.depth, first, job.rights);
-    atomicMax(&a.counts[3], job.depth);
-    const double ax[3] = {Rr[0], Rr[3], Rr[6]};
-    double s9[9];
-    for (int k = 0; k < 9; k++) s9[k] = acc[k];
-    box[0] = split_plane(s9, num, ax);
+    if (!a.refit) {
+      nd.depth = job.depth;
+      nd.first_child = dfs_first_child(job.depth, first, job.rights);
+      atomicMax(&a.counts[3], job.depth);
+      const double ax[3] = {Rr[0], Rr[3], Rr[6]};
+      double s9[9];
+      for (int k = 0; k < 9; k++) s9[k] = acc[k];
+      box[0] = split_plane(s9, num, ax);
+    }
   }
   __syncthreads();
+  if (a.refit) return;   // the tree stays: no split
 
   // split (split_tris, C2A_Build.cpp:344-385): the in-place partition swaps
   // each triangle at or below the plane with the first one above it, so the
@@ -458,6 +465,7 @@ __global__ void __launch_bounds__(64) build_thread(BuildArgs a, int32_t njobs) {
   const BuildJob j = a.jobs[2][t];
   double S1[3];
   fit_small(a, j, S1);
+  if (a.refit) return;   // the tree stays: no split
   GpuNode& nd = a.nodes[j.bn];
   atomicMax(&a.counts[3], j.depth);
   if (j.num == 1) {
@@ -522,6 +530,13 @@ __global__ void gather_tris(const double* src, const int32_t* ids, const int32_t
   for (int k = 0; k < 3; k++) vidx_out[3 * i + k] = vidx ? vidx[3 * (int64_t)s + k] : 3 * s + k;
 }
 
+__global__ void gather_xyz(const double* src, const int32_t* perm, int64_t n, double* tris) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t s = perm[i];
+  for (int k = 0; k < 9; k++) tris[9 * i + k] = src[9 * (int64_t)s + k];
+}
+
 __global__ void iota(int32_t* p, int64_t n) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) p[i] = (int32_t)i;
@@ -580,6 +595,7 @@ hipError_t build_bvh_device(const double* xyz9, const int32_t* ids, const int32_
   BVH_TRY(hipGetLastError());
 
   BuildArgs a;
+  memset(&a, 0, sizeof(a));   // refit = 0: a build
   a.src = (const double*)src.p;
   a.perm = (int32_t*)perm.p;
   a.perm_tmp = (int32_t*)perm_tmp.p;
@@ -632,6 +648,8 @@ hipError_t build_bvh_device(const double* xyz9, const int32_t* ids, const int32_
   out->tris.resize((size_t)n * 9);
   out->tri_id.resize((size_t)n);
   out->tri_vidx.resize((size_t)n * 3);
+  out->src_index.resize((size_t)n);
+  BVH_TRY(hipMemcpyAsync(out->src_index.data(), a.perm, (size_t)n * 4, hipMemcpyDeviceToHost, s));
   BVH_TRY(hipMemcpyAsync(out->nodes.data(), nodes, (size_t)nn * sizeof(GpuNode),
                          hipMemcpyDeviceToHost, s));
   BVH_TRY(hipMemcpyAsync(out->tris.data(), tris.p, (size_t)n * 9 * sizeof(double),
@@ -647,6 +665,68 @@ hipError_t build_bvh_device(const double* xyz9, const int32_t* ids, const int32_
   nodes_guard.p = nullptr;
   *tris_dev = tris.p;
   tris.p = nullptr;
+  return hipSuccess;
+}
+
+// Refit on the device: every node at once (the tree is known), each in its
+// size class, then the parent-relative frames and the triangles in build
+// order.  Copies the result back into *b.
+hipError_t refit_bvh_device(const double* xyz9, Bvh* b, void* nodes_dev, void* tris_dev,
+                            hipStream_t s) {
+  const int64_t n = (int64_t)b->tri_id.size();
+  const int64_t nn = 2 * n - 1;
+  if (n <= 0 || (int64_t)b->src_index.size() != n || (int64_t)b->nodes.size() != nn)
+    return hipErrorInvalidValue;
+  std::vector<int32_t> first, num, depth;
+  node_ranges(b->nodes, first, num, depth);
+  std::vector<BuildJob> jobs[3];
+  for (int64_t v = 0; v < nn; v++)
+    jobs[node_class(num[(size_t)v])].push_back(
+        BuildJob{(int32_t)v, first[(size_t)v], num[(size_t)v], depth[(size_t)v], 0, {0, 0, 0}});
+  Dev src, perm, cand, mRT, parent, lists[3];
+  BVH_TRY(src.alloc((size_t)n * 9 * sizeof(double)));
+  BVH_TRY(perm.alloc((size_t)n * 4));
+  BVH_TRY(cand.alloc((size_t)n * 12));
+  BVH_TRY(hipMemcpyAsync(src.p, xyz9, (size_t)n * 9 * sizeof(double), hipMemcpyHostToDevice, s));
+  BVH_TRY(hipMemcpyAsync(perm.p, b->src_index.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+  BuildArgs a;
+  memset(&a, 0, sizeof(a));
+  a.src = (const double*)src.p;
+  a.perm = (int32_t*)perm.p;
+  a.cand = (int32_t*)cand.p;
+  a.nodes = (GpuNode*)nodes_dev;
+  a.refit = 1;
+  for (int c = 0; c < 3; c++) {
+    if (jobs[c].empty()) continue;
+    BVH_TRY(lists[c].alloc(jobs[c].size() * sizeof(BuildJob)));
+    BVH_TRY(hipMemcpyAsync(lists[c].p, jobs[c].data(), jobs[c].size() * sizeof(BuildJob),
+                           hipMemcpyHostToDevice, s));
+    a.jobs[c] = (const BuildJob*)lists[c].p;
+  }
+  if (!jobs[0].empty())
+    hipLaunchKernelGGL(build_node<256>, dim3((unsigned)jobs[0].size()), dim3(256), 0, s, a, 0);
+  if (!jobs[1].empty())
+    hipLaunchKernelGGL(build_node<64>, dim3((unsigned)jobs[1].size()), dim3(64), 0, s, a, 1);
+  if (!jobs[2].empty())
+    hipLaunchKernelGGL(build_thread, dim3(grid_of((int64_t)jobs[2].size(), 64)), dim3(64), 0, s,
+                       a, (int32_t)jobs[2].size());
+  BVH_TRY(hipGetLastError());
+  BVH_TRY(mRT.alloc((size_t)nn * 12 * sizeof(double)));
+  BVH_TRY(parent.alloc((size_t)nn * 4));
+  hipLaunchKernelGGL(model_frames, dim3(grid_of(nn, 256)), dim3(256), 0, s, a.nodes, nn,
+                     (double*)mRT.p, (int32_t*)parent.p);
+  BVH_TRY(hipGetLastError());
+  hipLaunchKernelGGL(relative_frames, dim3(grid_of(nn, 256)), dim3(256), 0, s, a.nodes, nn,
+                     (const double*)mRT.p, (const int32_t*)parent.p);
+  BVH_TRY(hipGetLastError());
+  hipLaunchKernelGGL(gather_xyz, dim3(grid_of(n, 256)), dim3(256), 0, s, a.src, a.perm, n,
+                     (double*)tris_dev);
+  BVH_TRY(hipGetLastError());
+  BVH_TRY(hipMemcpyAsync(b->nodes.data(), nodes_dev, (size_t)nn * sizeof(GpuNode),
+                         hipMemcpyDeviceToHost, s));
+  BVH_TRY(hipMemcpyAsync(b->tris.data(), tris_dev, (size_t)n * 9 * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  BVH_TRY(hipStreamSynchronize(s));
   return hipSuccess;
 }
 

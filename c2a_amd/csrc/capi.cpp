@@ -566,10 +566,12 @@ uint64_t payload_digest(const Bvh& b) {
   uint64_t h = digest_bytes(0x50415931u, b.nodes.data(), b.nodes.size() * sizeof(GpuNode));
   h = digest_bytes(h, b.tris.data(), b.tris.size() * sizeof(double));
   h = digest_bytes(h, b.tri_id.data(), b.tri_id.size() * sizeof(int32_t));
-  return digest_bytes(h, b.tri_vidx.data(), b.tri_vidx.size() * sizeof(int32_t));
+  h = digest_bytes(h, b.tri_vidx.data(), b.tri_vidx.size() * sizeof(int32_t));
+  return digest_bytes(h, b.src_index.data(), b.src_index.size() * sizeof(int32_t));
 }
 
 const char kCacheMagic[8] = {'C', '2', 'A', 'B', 'V', 'H', 0, 1};
+constexpr uint32_t kCacheVersion = 2;   // 2: the source index of each triangle
 struct CacheHeader {
   char magic[8];
   uint32_t version, node_bytes;
@@ -651,7 +653,7 @@ int c2a_mesh_save(const c2a_mesh* m, const char* path) {
   CacheHeader h;
   memset(&h, 0, sizeof(h));
   memcpy(h.magic, kCacheMagic, 8);
-  h.version = 1;
+  h.version = kCacheVersion;
   h.node_bytes = (uint32_t)sizeof(GpuNode);
   h.ntri = m->ntri;
   h.max_depth = m->host.max_depth;
@@ -668,6 +670,7 @@ int c2a_mesh_save(const c2a_mesh* m, const char* path) {
             (std::streamsize)(m->host.tris.size() * sizeof(double)));
     f.write((const char*)m->host.tri_id.data(), (std::streamsize)(m->host.tri_id.size() * 4));
     f.write((const char*)m->host.tri_vidx.data(), (std::streamsize)(m->host.tri_vidx.size() * 4));
+    f.write((const char*)m->host.src_index.data(), (std::streamsize)(m->host.src_index.size() * 4));
     if (!f) return fail(C2A_ERR_BAD_ARG, "write failed: " + tmp);
   }
   // written under a temporary name, then moved: readers never see half a file
@@ -683,7 +686,7 @@ int c2a_mesh_load(const char* path, const double* tri_xyz9, const int32_t* tri_i
   std::ifstream f(path, std::ios::binary);
   if (!f) return fail(C2A_ERR_CACHE, std::string("cannot open ") + path);
   CacheHeader h;
-  if (!f.read((char*)&h, sizeof(h)) || memcmp(h.magic, kCacheMagic, 8) != 0 || h.version != 1 ||
+  if (!f.read((char*)&h, sizeof(h)) || memcmp(h.magic, kCacheMagic, 8) != 0 || h.version != kCacheVersion ||
       h.node_bytes != sizeof(GpuNode) || h.ntri <= 0 || h.ntri > (1 << 28))
     return fail(C2A_ERR_CACHE, std::string("not a hierarchy file: ") + path);
   if (tri_xyz9 &&
@@ -697,6 +700,7 @@ int c2a_mesh_load(const char* path, const double* tri_xyz9, const int32_t* tri_i
     m->host.tris.resize((size_t)n * 9);
     m->host.tri_id.resize((size_t)n);
     m->host.tri_vidx.resize((size_t)n * 3);
+    m->host.src_index.resize((size_t)n);
   } catch (const std::bad_alloc&) {
     delete m;
     return fail(C2A_ERR_OOM, "host allocation failed");
@@ -705,6 +709,7 @@ int c2a_mesh_load(const char* path, const double* tri_xyz9, const int32_t* tri_i
   f.read((char*)m->host.tris.data(), (std::streamsize)(m->host.tris.size() * sizeof(double)));
   f.read((char*)m->host.tri_id.data(), (std::streamsize)(m->host.tri_id.size() * 4));
   f.read((char*)m->host.tri_vidx.data(), (std::streamsize)(m->host.tri_vidx.size() * 4));
+  f.read((char*)m->host.src_index.data(), (std::streamsize)(m->host.src_index.size() * 4));
   bool ok = (bool)f;
   char extra;
   if (ok && f.read(&extra, 1)) ok = false;   // trailing bytes: not this format
@@ -716,6 +721,40 @@ int c2a_mesh_load(const char* path, const double* tri_xyz9, const int32_t* tri_i
   m->ntri = n;
   m->src_digest = h.src_digest;
   *out = m;
+  return C2A_OK;
+}
+
+int c2a_mesh_refit(c2a_mesh* m, const double* tri_xyz9, int32_t on_device, void* stream) {
+  if (!m || !tri_xyz9) return fail(C2A_ERR_BAD_ARG, "null argument");
+  if ((int64_t)m->host.src_index.size() != m->ntri)
+    return fail(C2A_ERR_BAD_ARG, "mesh has no source order");
+  int rc;
+  if (on_device) {
+    if ((rc = check_device()) || (rc = ensure_uploaded(m))) return rc;
+    hipError_t e;
+    try {
+      e = refit_bvh_device(tri_xyz9, &m->host, m->nodes.p, m->tris.p, (hipStream_t)stream);
+    } catch (const std::bad_alloc&) {
+      return fail(C2A_ERR_OOM, "host allocation failed during the refit");
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return e == hipErrorOutOfMemory ? fail(C2A_ERR_OOM, "device memory exhausted by the refit")
+                                      : hip_fail(e, "device refit");
+    }
+  } else {
+    if (refit_bvh(&m->host, tri_xyz9) != 0) return fail(C2A_ERR_BAD_ARG, "refit failed");
+    if (m->nodes.p) {   // keep the device copy current
+      HIP_TRY(hipMemcpy(m->nodes.p, m->host.nodes.data(), m->host.nodes.size() * sizeof(GpuNode),
+                        hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(m->tris.p, m->host.tris.data(), m->host.tris.size() * sizeof(double),
+                        hipMemcpyHostToDevice));
+    }
+  }
+  // a refitted hierarchy is not the build of these triangles: its cache
+  // file matches no build input
+  m->src_digest = digest_bytes(m->src_digest ^ 0x5245464954ull, tri_xyz9,
+                               (size_t)m->ntri * 9 * sizeof(double));
   return C2A_OK;
 }
 

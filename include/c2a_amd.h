@@ -184,6 +184,19 @@ int c2a_mesh_create_device(const double *tri_xyz9, const int32_t *tri_ids,
                            const int32_t *vert_idx3, int64_t ntri, void *stream,
                            c2a_mesh **out);
 
+/* Refit (no counterpart in the reference, which rebuilds: C2A_PQP.cpp:331-417):
+ * keeps the tree, the triangle order and the node numbers, and fits every
+ * node's RSS volume again -- with the build's fit: covariance orientation,
+ * rectangle and radius, parent-relative frames -- to moved triangles
+ * tri_xyz9 (ntri x 9 doubles, the same triangles in the order given at
+ * creation).  Each node is fitted over its triangles in their final order,
+ * so a refit at unchanged triangles equals the build up to the order of the
+ * fit's sums and corner pass.  on_device = 1 runs it on the GPU (and keeps
+ * the device copy), 0 on the host.  For deforming meshes: the volumes stay
+ * conservative, the tree's quality degrades with the motion.  A refitted
+ * mesh's cache file matches no build input. */
+int c2a_mesh_refit(c2a_mesh *m, const double *tri_xyz9, int32_t on_device, void *stream);
+
 /* On-disk cache of a built hierarchy (no counterpart in the reference, which
  * rebuilds in every EndModel, C2A_PQP.cpp:331-417).  The file holds the
  * nodes, the triangles in build order with their ids and vertex indices, a
