@@ -19,6 +19,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "bvh.h"
@@ -668,9 +669,10 @@ hipError_t build_bvh_device(const double* xyz9, const int32_t* ids, const int32_
   return hipSuccess;
 }
 
-// Refit on the device: every node at once (the tree is known), each in its
-// size class, then the parent-relative frames and the triangles in build
-// order.  Copies the result back into *b.
+// Refit on the device: the tree is known, so each depth's nodes go in one
+// launch per size class (no host round trip between levels), then the
+// parent-relative frames and the triangles in build order.  Copies the
+// result back into *b.
 hipError_t refit_bvh_device(const double* xyz9, Bvh* b, void* nodes_dev, void* tris_dev,
                             hipStream_t s) {
   const int64_t n = (int64_t)b->tri_id.size();
@@ -679,10 +681,28 @@ hipError_t refit_bvh_device(const double* xyz9, Bvh* b, void* nodes_dev, void* t
     return hipErrorInvalidValue;
   std::vector<int32_t> first, num, depth;
   node_ranges(b->nodes, first, num, depth);
+  // level by level: a node's corner candidates live in its triangle range of
+  // a.cand, which its descendants' ranges overlap, so only nodes of one
+  // depth (disjoint ranges) may run together
+  int max_depth = 0;
+  for (int64_t v = 0; v < nn; v++) max_depth = std::max(max_depth, (int)depth[(size_t)v]);
   std::vector<BuildJob> jobs[3];
-  for (int64_t v = 0; v < nn; v++)
-    jobs[node_class(num[(size_t)v])].push_back(
-        BuildJob{(int32_t)v, first[(size_t)v], num[(size_t)v], depth[(size_t)v], 0, {0, 0, 0}});
+  // jobs of each class ordered by depth, with per-depth offsets
+  std::vector<std::vector<int32_t>> cnt(3, std::vector<int32_t>((size_t)max_depth + 1, 0));
+  for (int64_t v = 0; v < nn; v++) cnt[node_class(num[(size_t)v])][(size_t)depth[(size_t)v]]++;
+  std::vector<std::vector<int32_t>> off(3, std::vector<int32_t>((size_t)max_depth + 2, 0));
+  for (int c = 0; c < 3; c++) {
+    for (int d = 0; d <= max_depth; d++) off[c][(size_t)d + 1] = off[c][(size_t)d] + cnt[c][(size_t)d];
+    jobs[c].resize((size_t)off[c][(size_t)max_depth + 1]);
+  }
+  {
+    std::vector<std::vector<int32_t>> fill = off;
+    for (int64_t v = 0; v < nn; v++) {
+      const int c = node_class(num[(size_t)v]);
+      jobs[c][(size_t)fill[c][(size_t)depth[(size_t)v]]++] =
+          BuildJob{(int32_t)v, first[(size_t)v], num[(size_t)v], depth[(size_t)v], 0, {0, 0, 0}};
+    }
+  }
   Dev src, perm, cand, mRT, parent, lists[3];
   BVH_TRY(src.alloc((size_t)n * 9 * sizeof(double)));
   BVH_TRY(perm.alloc((size_t)n * 4));
@@ -701,16 +721,17 @@ hipError_t refit_bvh_device(const double* xyz9, Bvh* b, void* nodes_dev, void* t
     BVH_TRY(lists[c].alloc(jobs[c].size() * sizeof(BuildJob)));
     BVH_TRY(hipMemcpyAsync(lists[c].p, jobs[c].data(), jobs[c].size() * sizeof(BuildJob),
                            hipMemcpyHostToDevice, s));
-    a.jobs[c] = (const BuildJob*)lists[c].p;
   }
-  if (!jobs[0].empty())
-    hipLaunchKernelGGL(build_node<256>, dim3((unsigned)jobs[0].size()), dim3(256), 0, s, a, 0);
-  if (!jobs[1].empty())
-    hipLaunchKernelGGL(build_node<64>, dim3((unsigned)jobs[1].size()), dim3(64), 0, s, a, 1);
-  if (!jobs[2].empty())
-    hipLaunchKernelGGL(build_thread, dim3(grid_of((int64_t)jobs[2].size(), 64)), dim3(64), 0, s,
-                       a, (int32_t)jobs[2].size());
-  BVH_TRY(hipGetLastError());
+  for (int d = 0; d <= max_depth; d++) {
+    for (int c = 0; c < 3; c++)
+      a.jobs[c] = (const BuildJob*)lists[c].p + off[c][(size_t)d];
+    const int32_t n0 = cnt[0][(size_t)d], n1 = cnt[1][(size_t)d], n2 = cnt[2][(size_t)d];
+    if (n0 > 0) hipLaunchKernelGGL(build_node<256>, dim3((unsigned)n0), dim3(256), 0, s, a, 0);
+    if (n1 > 0) hipLaunchKernelGGL(build_node<64>, dim3((unsigned)n1), dim3(64), 0, s, a, 1);
+    if (n2 > 0)
+      hipLaunchKernelGGL(build_thread, dim3(grid_of((int64_t)n2, 64)), dim3(64), 0, s, a, n2);
+    BVH_TRY(hipGetLastError());
+  }
   BVH_TRY(mRT.alloc((size_t)nn * 12 * sizeof(double)));
   BVH_TRY(parent.alloc((size_t)nn * 4));
   hipLaunchKernelGGL(model_frames, dim3(grid_of(nn, 256)), dim3(256), 0, s, a.nodes, nn,
