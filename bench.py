@@ -369,7 +369,7 @@ def main():
             "order": ("plain (reference DFS order, bit-exact)" if args.order != "learned"
                       else "learned (scorer-ordered siblings)"),
             "nq": nq,
-            "spec_depth": args.spec_depth or 5,
+            "spec_depth": args.spec_depth or "autotuned per mesh-size bucket (c2a_amd/tuning.json)",
             "bv_tests": int(res["bv_tests"].sum()),
             "tri_tests": int(res["tri_tests"].sum()),
             "ca_steps": int(res["numCA"].sum()),
